@@ -1,0 +1,139 @@
+"""Benchmark: path-steps/s of the fused Heston Andersen-QE path kernel, FP32 vs FP64.
+
+Workload (one "step" = one full pricing): Heston QE (psi_c = 1.5, martingale
+correction), 4,000,000 paths x 365 steps, S0=100, K=100, T=1, r=0.03, q=0,
+v0=0.04, kappa=2, theta=0.04, sigma=0.5, rho=-0.7, seed 42, three payoffs from
+one path set: arithmetic-average Asian call K=100, up-and-out call K=100 B=130
+monitored daily, floating-strike lookback put.  Inputs are resident on the
+device (the kernel generates its own random numbers); the timed region is K
+back-to-back pricings bracketed by device synchronisations.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "path-steps/s Heston QE FP32 vs FP64 at ≤1 SE error; p50/p99 pricing latency"
+N_PATHS, N_STEPS = 4_000_000, 365
+# Algorithmic work per path-step of the QE kernel with three payoffs: 46 flops
+# (quadratic branch incl. Box-Muller) + 3 payoff-tracking ops; 17 transcendentals
+# (log, sqrt x4, sin, cos, exp, log, rcp x5 ...) counted separately.
+FLOPS_PER_PS = 49
+TRANS_PER_PS = 17
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def workload(precision, n_paths=N_PATHS):
+    import quantkos_amd as qk
+    c = qk.contract(model=qk.HESTON, scheme=qk.QE, precision=precision, n_steps=N_STEPS,
+                    n_paths=n_paths, seed=42, S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0,
+                    theta=0.04, sigma=0.5, rho=-0.7, price_scale=100.0)
+    pays = [qk.payoff(qk.ASIAN, qk.CALL, 1.0), qk.payoff(qk.BARRIER_UP_OUT, qk.CALL, 1.0, 1.3),
+            qk.payoff(qk.LOOKBACK_FLOATING, qk.PUT, 1.0)]
+    return c, pays
+
+
+def run(engine, precision, steps, warmup):
+    c, pays = workload(precision)
+    for _ in range(warmup):
+        engine.price_payoffs(c, pays)
+    engine.synchronize()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = engine.price_payoffs(c, pays)
+        kern.append(engine.timing().path_kernel_ms)
+    engine.synchronize()
+    dt = time.perf_counter() - t0
+    return res, dt, kern
+
+
+def cpu_baseline(target_s=15.0):
+    """The FP64 CPU oracle (a port of the reference's per-path algorithm, OpenMP)
+    on a bounded sample of the same workload, on this host's cores."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from oracle import oracle as o
+    import quantkos_amd as qk
+    c, pays = workload(qk.FP64)
+    mk = lambda n: o.model(model=0, scheme=2, fp32_semantics=0, n_steps=N_STEPS, seed=42, S0=1.0,
+                           v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5,
+                           rho=-0.7, lambda_j=0.0, mu_j=0.0, sigma_j=0.0)
+    op = [o.Payoff(p.type, p.right, p.K, p.B) for p in pays]
+    n = 4096 * cores
+    t0 = time.perf_counter()
+    o.price(mk(n), n, op, 100.0)
+    cal = time.perf_counter() - t0
+    n2 = int(max(n, min(4_000_000, n * target_s / max(cal, 1e-3))))
+    t0 = time.perf_counter()
+    o.price(mk(n2), n2, op, 100.0)
+    el = time.perf_counter() - t0
+    return {"value": n2 * N_STEPS / el, "unit": "path-steps/s", "cores": cores, "kind": "port",
+            "sample": "%d paths x %d steps of the same QE workload (3 payoffs), FP64, %.1f s" % (n2, N_STEPS, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import quantkos_amd as qk
+    eng = qk.Engine(0)
+    r32, dt32, k32 = run(eng, qk.FP32, args.steps, args.warmup)
+    s64 = args.fp64_steps or max(1, args.steps // 2)
+    r64, dt64, k64 = run(eng, qk.FP64, s64, 1)
+    eng.close()
+    ps = N_PATHS * N_STEPS
+    v32 = ps * args.steps / dt32
+    v64 = ps * s64 / dt64
+    kavg32 = sum(k32) / len(k32)
+    kavg64 = sum(k64) / len(k64)
+    ach = FLOPS_PER_PS * ps / (kavg32 * 1e-3) / 1e12
+    err_se = [abs(a.price - b.price) / b.se for a, b in zip(r32, r64)]
+    line = {
+        "metric": METRIC, "value": v32, "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt32 * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "Heston Andersen-QE (psi_c=1.5, martingale correction), 4,000,000 paths x "
+                               "365 steps, 3 payoffs from one path set: Asian call K=100, up-and-out "
+                               "call K=100 B=130 daily, floating-strike lookback put",
+                   "paths": N_PATHS, "time_steps": N_STEPS, "parallelism": "1 GPU"},
+        "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_paths<float,Heston,QE,pathdep>", "kernel_ms": kavg32,
+                     "flops_per_path_step": FLOPS_PER_PS, "transcendentals_per_path_step": TRANS_PER_PS},
+        "fp64": {"value": v64, "ms_per_step": dt64 * 1e3 / s64, "kernel_ms": kavg64, "steps": s64,
+                 "achieved_tflops": FLOPS_PER_PS * ps / (kavg64 * 1e-3) / 1e12,
+                 "peak": PEAK_FP64_TFLOPS},
+        "fp32_over_fp64": v32 / v64,
+        "prices": {"fp32": [round(a.price, 6) for a in r32], "fp64": [round(b.price, 6) for b in r64],
+                   "se_fp64": [round(b.se, 6) for b in r64], "fp32_err_in_se": [round(e, 4) for e in err_se]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+/*
+ * qk.h -- C ABI of the MI355X-native QuantKos Monte Carlo engine (libquantkos.so).
+ *
+ * Plain C types only: no C++ or torch types cross this boundary, no exception
+ * escapes it.  Every call returns a qk_status; qk_last_error() gives the text.
+ * The caller owns every input and output struct; a qk_ctx owns all device
+ * memory (path stores, partial sums, request queues) and one HIP stream.
+ * Calls on one ctx are not thread-safe, except the qk_server_* family, whose
+ * submit/poll/wait are internally synchronised.
+ *
+ * The reference (QuantKos) has no C ABI: its operator API is a set of C++
+ * template policies.  Each entry point below names the reference interface it
+ * replaces; the C++ drop-in types in quantkos_amd/host/ map onto these calls.
+ */
+#ifndef QK_H
+#define QK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    QK_OK = 0,
+    QK_EINVAL = -1,     /* invalid argument (message in qk_last_error)            */
+    QK_ENOMEM = -2,     /* request exceeds the ctx memory budget or device memory  */
+    QK_EDEVICE = -3,    /* HIP runtime error                                       */
+    QK_EAGAIN = -4,     /* server queue full; retry later                          */
+    QK_ENOTFOUND = -5,  /* unknown ticket                                          */
+    QK_ENOTREADY = -6   /* ticket not finished yet (qk_server_poll)                */
+} qk_status;
+
+/* Enumerations in the order of the reference's ConfigFileEnums.hpp:24-58. */
+enum { QK_MODEL_HESTON = 0, QK_MODEL_BATES = 1 };
+enum { QK_SCHEME_EULER = 0, QK_SCHEME_MILSTEIN = 1, QK_SCHEME_QE = 2 };
+enum {
+    QK_EUROPEAN = 0, QK_ASIAN = 1, QK_BARRIER_UP_OUT = 2, QK_BARRIER_DOWN_OUT = 3,
+    QK_BARRIER_UP_IN = 4, QK_BARRIER_DOWN_IN = 5, QK_LOOKBACK_FLOATING = 6,
+    QK_LOOKBACK_FIXED = 7, QK_BINARY_CASH = 8, QK_BINARY_ASSET = 9, QK_AMERICAN = 10
+};
+enum { QK_CALL = 0, QK_PUT = 1 };
+enum { QK_FP32 = 0, QK_FP64 = 1 };
+
+typedef struct qk_ctx qk_ctx;
+
+/* One model + grid + path set.  Monetary inputs are taken as given: the host
+ * layer applies the reference's Normalize_prices rule (S0 -> 1, K,B /= S0;
+ * Config.hpp:710-723) and passes the scale in price_scale, by which every
+ * price and SE is multiplied (except cash-or-nothing binaries, scale 1;
+ * OptionPricer.hpp:323-332).  dt = T / n_steps. */
+typedef struct {
+    int32_t model, scheme, precision, n_steps;
+    int64_t n_paths;
+    uint64_t seed;
+    double S0, v0, r, q, T;
+    double kappa, theta, sigma, rho;
+    double lambda_j, mu_j, sigma_j;
+    double price_scale;
+} qk_contract;
+
+/* Payoff spec (PayoffTracker<Type,Right>, Payoff.hpp:84-243). */
+typedef struct { int32_t type, right; double K, B; } qk_payoff;
+
+/* Pricing result (MCOpPrices, OptionPricer.hpp:50-70) plus the raw moments. */
+typedef struct {
+    double price, se, prob_itm, mean, m2;
+    int64_t n, itm;
+} qk_stats;
+
+/* Greek stencil switches and bump sizes (MCConfig, Config.hpp:431-447). */
+typedef struct {
+    int32_t delta_gamma, vega_vomma, vanna, rho, theta;
+    int32_t theta_central;   /* 0: reference one-sided (T-h); 1: central (T-h, T+h) */
+    double rel_S, abs_vol, abs_r, abs_T;
+} qk_bumps;
+
+/* MCGreeks (MCEngine.hpp:93-101) plus the base price and the lane count. */
+typedef struct {
+    double delta, gamma, vega, vomma, vanna, rho, theta;
+    double base_price, base_se;
+    int32_t n_lanes;
+} qk_greeks;
+
+/* Device-time breakdown of the last call on a ctx (HIP events on its stream). */
+typedef struct {
+    double path_kernel_ms;   /* fused path-generation kernel(s)             */
+    double reduce_ms;        /* tile-partial merge + finalisation           */
+    double lsm_backward_ms;  /* Longstaff-Schwartz backward sweep (American) */
+    double total_ms;         /* first to last event of the call             */
+    int64_t path_steps;      /* path-steps simulated by the call            */
+} qk_timing;
+
+/* ---- context ---- */
+/* Replaces Kokkos::initialize + the memory managers' budget (PathsMCBatchMem.hpp:49-421):
+ * binds a device and a byte budget that every call's device footprint must fit. */
+qk_status qk_ctx_create(int device, size_t mem_budget_bytes, qk_ctx **out);
+qk_status qk_ctx_destroy(qk_ctx *ctx);
+const char *qk_strerror(qk_status s);
+const char *qk_last_error(const qk_ctx *ctx);
+qk_status qk_synchronize(qk_ctx *ctx);
+qk_status qk_last_timing(const qk_ctx *ctx, qk_timing *out);
+/* Bytes of device memory a call with this contract / payload would need. */
+size_t qk_footprint_bytes(const qk_contract *c, int32_t n_lanes, int32_t n_payoffs, int32_t american);
+
+/* ---- pricing ---- */
+/* Plain single-contract launch.  Replaces MSolver<M,S,T,R>::execute_batch
+ * (MSolver.hpp:195-232) + OptionPricer::accumulate_batch_payoffs /
+ * evaluate_option_price (OptionPricer.hpp:123-249) for one OptType. */
+qk_status qk_price(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoff, qk_stats *out);
+
+/* Several payoffs from one path set in one launch (the reference needs one run
+ * per OptType, Distpatcher.hpp:140-178).  out[n_payoffs]. */
+qk_status qk_price_payoffs(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoffs,
+                           int32_t n_payoffs, qk_stats *out);
+
+/* Lanes = contracts that share model, scheme, precision, n_steps, n_paths and
+ * seed, and therefore the random stream (common random numbers); they may
+ * differ in S0, v0, r, q, T and the Heston/jump parameters.  One launch
+ * prices every lane x payoff.  out[n_lanes * n_payoffs], lane-major. */
+qk_status qk_price_lanes(qk_ctx *ctx, const qk_contract *lanes, int32_t n_lanes,
+                         const qk_payoff *payoffs, int32_t n_payoffs, qk_stats *out);
+
+/* Finite-difference Greeks with the reference's stencils (MCEngine.hpp:576-735),
+ * every bump a lane of one launch; differences taken in FP64 on the device.
+ * out[n_payoffs]. */
+qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoffs, int32_t n_payoffs,
+                          const qk_bumps *bumps, qk_greeks *out);
+
+/* American option by Longstaff-Schwartz with the unweighted Laguerre basis of
+ * degree 3 on in-the-money paths, exercise at every grid date.  Replaces
+ * ForwardBackwardMCRunner (ForwardBackwardMC.hpp:68-256) + LSMEngine
+ * (LSMAmerican.hpp:217-614); the path store stays in HBM.  coeffs_out, if not
+ * NULL, receives the regression coefficients [n_steps][4] (date t row = the
+ * regression at date index t; row n_steps-1 is zero). */
+qk_status qk_price_american(qk_ctx *ctx, const qk_contract *c, int32_t right, double K,
+                            qk_stats *out, double *coeffs_out);
+
+/* Simulated spot paths, copied to host memory laid out [step][path]
+ * (S after step t+1 at row t), as the reference's StorePaths kernel writes
+ * (MSolver.hpp:117-119). */
+qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, double *S_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+"""ctypes binding of libquantkos.so (include/qk.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading fails
+loudly if the in-tree build is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libquantkos.so")
+
+QK_OK, QK_EINVAL, QK_ENOMEM, QK_EDEVICE, QK_EAGAIN, QK_ENOTFOUND, QK_ENOTREADY = 0, -1, -2, -3, -4, -5, -6
+HESTON, BATES = 0, 1
+EULER, MILSTEIN, QE = 0, 1, 2
+(EUROPEAN, ASIAN, BARRIER_UP_OUT, BARRIER_DOWN_OUT, BARRIER_UP_IN, BARRIER_DOWN_IN,
+ LOOKBACK_FLOATING, LOOKBACK_FIXED, BINARY_CASH, BINARY_ASSET, AMERICAN) = range(11)
+CALL, PUT = 0, 1
+FP32, FP64 = 0, 1
+
+
+class Contract(C.Structure):
+    _fields_ = [("model", C.c_int32), ("scheme", C.c_int32), ("precision", C.c_int32),
+                ("n_steps", C.c_int32), ("n_paths", C.c_int64), ("seed", C.c_uint64),
+                ("S0", C.c_double), ("v0", C.c_double), ("r", C.c_double), ("q", C.c_double),
+                ("T", C.c_double), ("kappa", C.c_double), ("theta", C.c_double),
+                ("sigma", C.c_double), ("rho", C.c_double), ("lambda_j", C.c_double),
+                ("mu_j", C.c_double), ("sigma_j", C.c_double), ("price_scale", C.c_double)]
+
+
+class Payoff(C.Structure):
+    _fields_ = [("type", C.c_int32), ("right", C.c_int32), ("K", C.c_double), ("B", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("price", C.c_double), ("se", C.c_double), ("prob_itm", C.c_double),
+                ("mean", C.c_double), ("m2", C.c_double), ("n", C.c_int64), ("itm", C.c_int64)]
+
+
+class Bumps(C.Structure):
+    _fields_ = [("delta_gamma", C.c_int32), ("vega_vomma", C.c_int32), ("vanna", C.c_int32),
+                ("rho", C.c_int32), ("theta", C.c_int32), ("theta_central", C.c_int32),
+                ("rel_S", C.c_double), ("abs_vol", C.c_double), ("abs_r", C.c_double),
+                ("abs_T", C.c_double)]
+
+
+class Greeks(C.Structure):
+    _fields_ = [("delta", C.c_double), ("gamma", C.c_double), ("vega", C.c_double),
+                ("vomma", C.c_double), ("vanna", C.c_double), ("rho", C.c_double),
+                ("theta", C.c_double), ("base_price", C.c_double), ("base_se", C.c_double),
+                ("n_lanes", C.c_int32)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("path_kernel_ms", C.c_double), ("reduce_ms", C.c_double),
+                ("lsm_backward_ms", C.c_double), ("total_ms", C.c_double),
+                ("path_steps", C.c_int64)]
+
+
+_lib = None
+
+
+def load():
+    """Loads the in-tree libquantkos.so and declares its C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libquantkos.so is not built (%s); run `python -m quantkos_amd.build`"
+                           % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    st = C.c_int
+    sig = {
+        "qk_ctx_create": ([C.c_int, C.c_size_t, P(vp)], st),
+        "qk_ctx_destroy": ([vp], st),
+        "qk_strerror": ([C.c_int], C.c_char_p),
+        "qk_last_error": ([vp], C.c_char_p),
+        "qk_synchronize": ([vp], st),
+        "qk_last_timing": ([vp, P(Timing)], st),
+        "qk_footprint_bytes": ([P(Contract), C.c_int32, C.c_int32, C.c_int32], C.c_size_t),
+        "qk_price": ([vp, P(Contract), P(Payoff), P(Stats)], st),
+        "qk_price_payoffs": ([vp, P(Contract), P(Payoff), C.c_int32, P(Stats)], st),
+        "qk_price_lanes": ([vp, P(Contract), C.c_int32, P(Payoff), C.c_int32, P(Stats)], st),
+        "qk_price_greeks": ([vp, P(Contract), P(Payoff), C.c_int32, P(Bumps), P(Greeks)], st),
+        "qk_price_american": ([vp, P(Contract), C.c_int32, C.c_double, P(Stats), P(C.c_double)], st),
+        "qk_simulate_paths": ([vp, P(Contract), P(C.c_double)], st),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L

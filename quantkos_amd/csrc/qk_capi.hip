@@ -1,0 +1,616 @@
+// qk_capi.hip -- the C ABI (include/qk.h): context, device buffers, argument
+// validation, host precomputation of scheme constants, and the pricing entry
+// points.  Host code only; the kernels live in qk_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qk_capi_internal.hpp"
+#include "qk_launch.hpp"
+
+using namespace qk;
+
+// ---------------------------------------------------------------------------
+// Errors.
+// ---------------------------------------------------------------------------
+extern "C" const char *qk_strerror(qk_status s) {
+    switch (s) {
+    case QK_OK: return "ok";
+    case QK_EINVAL: return "invalid argument";
+    case QK_ENOMEM: return "out of memory budget";
+    case QK_EDEVICE: return "device error";
+    case QK_EAGAIN: return "queue full, retry";
+    case QK_ENOTFOUND: return "unknown ticket";
+    case QK_ENOTREADY: return "not ready";
+    }
+    return "unknown status";
+}
+
+extern "C" const char *qk_last_error(const qk_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+namespace qk {
+
+qk_status fail(qk_ctx *ctx, qk_status s, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return s;
+}
+
+qk_status hip_fail(qk_ctx *ctx, hipError_t e, const char *what) {
+    return fail(ctx, QK_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define QK_HIP(call)                                  \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+qk_status DevBuf::reserve(qk_ctx *ctx, size_t bytes) {
+    if (bytes <= cap) return QK_OK;
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipFree");
+    }
+    const size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(ctx, QK_ENOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed: " + hipGetErrorString(e));
+    }
+    cap = want;
+    return QK_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Validation.
+// ---------------------------------------------------------------------------
+qk_status validate_contract(qk_ctx *ctx, const qk_contract *c) {
+    if (!c) return fail(ctx, QK_EINVAL, "null contract");
+    if (c->model < QK_MODEL_HESTON || c->model > QK_MODEL_BATES) return fail(ctx, QK_EINVAL, "contract.model out of range");
+    if (c->scheme < QK_SCHEME_EULER || c->scheme > QK_SCHEME_QE) return fail(ctx, QK_EINVAL, "contract.scheme out of range");
+    if (c->precision != QK_FP32 && c->precision != QK_FP64) return fail(ctx, QK_EINVAL, "contract.precision out of range");
+    if (c->n_steps < 1) return fail(ctx, QK_EINVAL, "contract.n_steps must be >= 1");
+    if (c->n_paths < 1) return fail(ctx, QK_EINVAL, "contract.n_paths must be >= 1");
+    if (!(c->T > 0.0) || !std::isfinite(c->T)) return fail(ctx, QK_EINVAL, "contract.T must be positive");
+    if (!(c->S0 > 0.0) || !std::isfinite(c->S0)) return fail(ctx, QK_EINVAL, "contract.S0 must be positive");
+    if (!(c->v0 >= 0.0)) return fail(ctx, QK_EINVAL, "contract.v0 must be >= 0");
+    if (!(c->kappa >= 0.0) || !(c->theta >= 0.0) || !(c->sigma >= 0.0))
+        return fail(ctx, QK_EINVAL, "contract kappa, theta, sigma must be >= 0");
+    if (!(c->rho >= -1.0 && c->rho <= 1.0)) return fail(ctx, QK_EINVAL, "contract.rho must be in [-1, 1]");
+    if (c->model == QK_MODEL_BATES && (!(c->lambda_j >= 0.0) || !(c->sigma_j >= 0.0) || !std::isfinite(c->mu_j)))
+        return fail(ctx, QK_EINVAL, "Bates lambda_j and sigma_j must be >= 0");
+    if (c->model == QK_MODEL_BATES && c->lambda_j * (c->T / c->n_steps) > 8.0)
+        return fail(ctx, QK_EINVAL, "Bates lambda_j * dt must be <= 8");
+    if (!(c->price_scale > 0.0)) return fail(ctx, QK_EINVAL, "contract.price_scale must be positive");
+    if (c->n_paths > ((int64_t)1 << 40)) return fail(ctx, QK_EINVAL, "contract.n_paths too large");
+    return QK_OK;
+}
+
+qk_status validate_payoff(qk_ctx *ctx, const qk_payoff *p) {
+    if (!p) return fail(ctx, QK_EINVAL, "null payoff");
+    if (p->type < QK_EUROPEAN || p->type > QK_BINARY_ASSET)
+        return fail(ctx, QK_EINVAL, "payoff.type out of range (American goes through qk_price_american)");
+    if (p->right != QK_CALL && p->right != QK_PUT) return fail(ctx, QK_EINVAL, "payoff.right out of range");
+    if (!(p->K > 0.0) || !std::isfinite(p->K)) return fail(ctx, QK_EINVAL, "payoff.K must be positive");
+    return QK_OK;
+}
+
+bool path_dependent(int type) { return !(type == QK_EUROPEAN || type == QK_BINARY_CASH || type == QK_BINARY_ASSET); }
+
+// ---------------------------------------------------------------------------
+// Scheme constants in FP64 (SDEScheme constructors, MSchemes.hpp:104-122,
+// :199-216, :296-352; Bates compensator :535-550).
+// ---------------------------------------------------------------------------
+template <typename Real>
+LaneC<Real> make_lane(const qk_contract &c) {
+    const double dt = c.T / (double)c.n_steps;
+    const double eps = c.precision == QK_FP32 ? 1e-6 : 1e-12;
+    double q = c.q, eml = 0.0;
+    if (c.model == QK_MODEL_BATES) {
+        const double kJ = std::exp(c.mu_j + 0.5 * c.sigma_j * c.sigma_j) - 1.0;
+        q += c.lambda_j * kJ;
+        eml = std::exp(-c.lambda_j * dt);
+    }
+    const double k = c.kappa, th = c.theta, sg = c.sigma;
+    double rho = c.rho;
+    LaneC<Real> L{};
+    L.S0 = (Real)c.S0;
+    L.v0 = (Real)c.v0;
+    L.rqdt = (Real)((c.r - q) * dt);
+    L.half_dt = (Real)(0.5 * dt);
+    L.sqrt_dt = (Real)std::sqrt(dt);
+    L.rho = (Real)rho;
+    L.rho_c = (Real)std::sqrt(1.0 - rho * rho);
+    L.omkdt = (Real)(1.0 - k * dt);
+    L.ktdt = (Real)(k * th * dt);
+    L.ssdt = (Real)(sg * std::sqrt(dt));
+    L.qs2dt = (Real)(0.25 * sg * sg * dt);
+    L.iden = (Real)(1.0 / (1.0 + k * dt));
+    const double ekdt = std::exp(-k * dt);
+    double tmf, s2f1, s2f2;
+    if (k < eps) {
+        tmf = 0.0;
+        s2f1 = sg * sg * dt;
+        s2f2 = 0.0;
+    } else {
+        const double ome = 1.0 - ekdt;
+        tmf = th * ome;
+        s2f1 = (sg * sg * ekdt / k) * ome;
+        s2f2 = (th * sg * sg / (2.0 * k)) * (ome * ome);
+    }
+    double rds;
+    if (sg < eps) {
+        rds = 0.0;
+        rho = 0.0;
+    } else {
+        rds = rho / sg;
+    }
+    const double omr2 = 1.0 - rho * rho;
+    const double K12 = k * rds - 0.5;
+    const double K2 = dt * 0.5 * K12 + rds, K3 = dt * 0.5 * omr2, K4 = dt * 0.5 * omr2;
+    L.ekdt = (Real)ekdt;
+    L.tmf = (Real)tmf;
+    L.s2f1 = (Real)s2f1;
+    L.s2f2 = (Real)s2f2;
+    L.K2 = (Real)K2;
+    L.K3 = (Real)K3;
+    L.K4 = (Real)K4;
+    L.A = (Real)(K2 + 0.5 * K4);
+    L.cv = (Real)(-0.5 * K3);
+    L.eps = (Real)eps;
+    L.eml_dt = (Real)eml;
+    L.mu_j = (Real)c.mu_j;
+    L.sigma_j = (Real)c.sigma_j;
+    return L;
+}
+
+template LaneC<float> make_lane<float>(const qk_contract &);
+template LaneC<double> make_lane<double>(const qk_contract &);
+
+// ---------------------------------------------------------------------------
+// Footprints (device bytes of one call), the budget test of admission.
+// ---------------------------------------------------------------------------
+size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
+    const size_t rs = c.precision == QK_FP32 ? 4 : 8;
+    const size_t n_tiles = (size_t)((c.n_paths + TILE - 1) / TILE);
+    const size_t groups = (size_t)n_lanes * std::max(n_pay, 1);
+    size_t b = groups * n_tiles * (8 + 8 + 4);          // tile moments
+    b += (size_t)(n_lanes + 4) * sizeof(LaneC<double>) + MAX_PAYOFFS * sizeof(PayD<double>);
+    b += groups * (sizeof(qk_stats) + 16) + 4096;
+    if (american) {
+        b += (size_t)c.n_paths * (size_t)c.n_steps * rs;   // path store [date][path]
+        b += (size_t)c.n_paths * rs;                         // cash flows
+        b += (size_t)c.n_steps * 4 * 8 + ((size_t)(c.n_paths + 255) / 256) * 16 * 8;
+    }
+    return b;
+}
+
+qk_status check_budget(qk_ctx *ctx, size_t bytes) {
+    if (bytes > ctx->budget)
+        return fail(ctx, QK_ENOMEM, "call needs " + std::to_string(bytes) + " device bytes, budget is " +
+                                        std::to_string(ctx->budget));
+    return QK_OK;
+}
+
+}  // namespace qk
+
+extern "C" size_t qk_footprint_bytes(const qk_contract *c, int32_t n_lanes, int32_t n_payoffs, int32_t american) {
+    if (!c) return 0;
+    return footprint(*c, std::max(n_lanes, 1), n_payoffs, american != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Context.
+// ---------------------------------------------------------------------------
+extern "C" qk_status qk_ctx_create(int device, size_t mem_budget_bytes, qk_ctx **out) {
+    if (!out) return QK_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return QK_EDEVICE;
+    if (device < 0 || device >= n) return QK_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return QK_EDEVICE;
+    qk_ctx *ctx = new qk_ctx();
+    ctx->device = device;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        delete ctx;
+        return QK_EDEVICE;
+    }
+    ctx->budget = mem_budget_bytes ? mem_budget_bytes : (size_t)(0.85 * (double)free_b);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return QK_EDEVICE;
+    }
+    for (auto &e : ctx->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return QK_EDEVICE;
+        }
+    }
+    if (ctx->counter.reserve(ctx, 256) != QK_OK || hipMemset(ctx->counter.p, 0, 256) != hipSuccess) {
+        delete ctx;
+        return QK_EDEVICE;
+    }
+    *out = ctx;
+    return QK_OK;
+}
+
+qk_ctx::~qk_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (DevBuf *b : {&lanes, &pays, &psum, &pm2, &pitm, &disc, &scale, &stats, &greeks, &store, &cf, &coeffs,
+                      &bpart, &counter, &sums})
+        b->release();
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" qk_status qk_ctx_destroy(qk_ctx *ctx) {
+    if (!ctx) return QK_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    delete ctx;
+    return QK_OK;
+}
+
+extern "C" qk_status qk_synchronize(qk_ctx *ctx) {
+    if (!ctx) return QK_EINVAL;
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    return QK_OK;
+}
+
+extern "C" qk_status qk_last_timing(const qk_ctx *ctx, qk_timing *out) {
+    if (!ctx || !out) return QK_EINVAL;
+    *out = ctx->timing;
+    return QK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lane pricing: the common core of qk_price / qk_price_payoffs /
+// qk_price_lanes / qk_price_greeks.  Leaves per-(lane, payoff) qk_stats in
+// ctx->stats on the device.
+// ---------------------------------------------------------------------------
+namespace qk {
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+    return ms;
+}
+
+template <typename Real>
+static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay) {
+    const qk_contract &c0 = lanes[0];
+    const bool pathdep = std::any_of(pays, pays + n_pay, [](const qk_payoff &p) { return path_dependent(p.type); });
+    const int L = n_lanes >= 4 ? 4 : 1;
+    const int n_padded = (n_lanes + L - 1) / L * L;
+    std::vector<LaneC<Real>> hl(n_padded);
+    std::vector<double> hdisc(n_lanes), hscale(n_pay);
+    for (int i = 0; i < n_padded; ++i) hl[i] = make_lane<Real>(lanes[std::min(i, n_lanes - 1)]);
+    for (int i = 0; i < n_lanes; ++i) hdisc[i] = std::exp(-lanes[i].r * lanes[i].T);
+    std::vector<PayD<Real>> hp(n_pay);
+    for (int j = 0; j < n_pay; ++j) {
+        hp[j] = PayD<Real>{pays[j].type, pays[j].right, (Real)pays[j].K, (Real)pays[j].B};
+        hscale[j] = pays[j].type == QK_BINARY_CASH ? 1.0 : c0.price_scale;
+    }
+    const int64_t n_tiles = (c0.n_paths + TILE - 1) / TILE;
+    const size_t groups = (size_t)n_lanes * n_pay;
+    qk_status st;
+    if ((st = ctx->lanes.reserve(ctx, hl.size() * sizeof(LaneC<Real>))) != QK_OK) return st;
+    if ((st = ctx->pays.reserve(ctx, hp.size() * sizeof(PayD<Real>))) != QK_OK) return st;
+    if ((st = ctx->disc.reserve(ctx, hdisc.size() * 8)) != QK_OK) return st;
+    if ((st = ctx->scale.reserve(ctx, hscale.size() * 8)) != QK_OK) return st;
+    if ((st = ctx->psum.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
+    if ((st = ctx->pm2.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
+    if ((st = ctx->pitm.reserve(ctx, groups * n_tiles * 4)) != QK_OK) return st;
+    if ((st = ctx->stats.reserve(ctx, groups * sizeof(qk_stats))) != QK_OK) return st;
+    hipStream_t s = ctx->stream;
+    QK_HIP(hipMemcpyAsync(ctx->lanes.p, hl.data(), hl.size() * sizeof(LaneC<Real>), hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(ctx->pays.p, hp.data(), hp.size() * sizeof(PayD<Real>), hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(ctx->disc.p, hdisc.data(), hdisc.size() * 8, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(ctx->scale.p, hscale.data(), hscale.size() * 8, hipMemcpyHostToDevice, s));
+
+    PathArgs<Real> a{};
+    a.lanes = (const LaneC<Real> *)ctx->lanes.p;
+    a.pays = (const PayD<Real> *)ctx->pays.p;
+    a.n_lanes = n_lanes;
+    a.n_pay = n_pay;
+    a.n_steps = c0.n_steps;
+    a.n_paths = c0.n_paths;
+    a.n_tiles = n_tiles;
+    a.key0 = (uint32_t)c0.seed;
+    a.key1 = (uint32_t)(c0.seed >> 32);
+    a.itm_eps = (Real)(c0.precision == QK_FP32 ? 1e-6 : 1e-12);   // OptionPricer.hpp:316
+    a.part_sum = (double *)ctx->psum.p;
+    a.part_m2 = (double *)ctx->pm2.p;
+    a.part_itm = (uint32_t *)ctx->pitm.p;
+    a.store = nullptr;
+
+    QK_HIP(hipEventRecord(ctx->ev[0], s));
+    QK_HIP(launch_paths<Real>(a, c0.model, c0.scheme, pathdep, false, L, s));
+    QK_HIP(hipEventRecord(ctx->ev[1], s));
+    QK_HIP(launch_merge(a.part_sum, a.part_m2, a.part_itm, n_tiles, c0.n_paths, n_lanes, n_pay,
+                        (const double *)ctx->disc.p, (const double *)ctx->scale.p, (qk_stats *)ctx->stats.p, s));
+    QK_HIP(hipEventRecord(ctx->ev[2], s));
+    return QK_OK;
+}
+
+static qk_status finish_timing(qk_ctx *ctx, int64_t path_steps, bool has_lsm) {
+    QK_HIP(hipEventSynchronize(ctx->ev[2]));
+    ctx->timing = qk_timing{};
+    ctx->timing.path_kernel_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+    if (has_lsm) {
+        ctx->timing.lsm_backward_ms = elapsed(ctx->ev[1], ctx->ev[3]);
+        ctx->timing.reduce_ms = elapsed(ctx->ev[3], ctx->ev[2]);
+    } else {
+        ctx->timing.reduce_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+    }
+    ctx->timing.total_ms = elapsed(ctx->ev[0], ctx->ev[2]);
+    ctx->timing.path_steps = path_steps;
+    return QK_OK;
+}
+
+static qk_status validate_lanes(qk_ctx *ctx, const qk_contract *lanes, int32_t n_lanes, const qk_payoff *pays,
+                                int32_t n_pay) {
+    if (!lanes || n_lanes < 1 || n_lanes > MAX_LANES) return fail(ctx, QK_EINVAL, "n_lanes must be in [1, 64]");
+    if (!pays || n_pay < 1 || n_pay > MAX_PAYOFFS) return fail(ctx, QK_EINVAL, "n_payoffs must be in [1, 16]");
+    for (int i = 0; i < n_lanes; ++i) {
+        qk_status st = validate_contract(ctx, &lanes[i]);
+        if (st != QK_OK) return st;
+        const qk_contract &a = lanes[0], &b = lanes[i];
+        if (a.model != b.model || a.scheme != b.scheme || a.precision != b.precision || a.n_steps != b.n_steps ||
+            a.n_paths != b.n_paths || a.seed != b.seed || a.price_scale != b.price_scale)
+            return fail(ctx, QK_EINVAL,
+                        "lanes must share model, scheme, precision, n_steps, n_paths, seed and price_scale");
+    }
+    for (int j = 0; j < n_pay; ++j) {
+        qk_status st = validate_payoff(ctx, &pays[j]);
+        if (st != QK_OK) return st;
+    }
+    return QK_OK;
+}
+
+}  // namespace qk
+
+extern "C" qk_status qk_price_lanes(qk_ctx *ctx, const qk_contract *lanes, int32_t n_lanes, const qk_payoff *pays,
+                                    int32_t n_pay, qk_stats *out) {
+    if (!ctx || !out) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_lanes(ctx, lanes, n_lanes, pays, n_pay);
+    if (st != QK_OK) return st;
+    if ((st = check_budget(ctx, footprint(lanes[0], n_lanes, n_pay, false))) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    st = lanes[0].precision == QK_FP32 ? run_lanes<float>(ctx, lanes, n_lanes, pays, n_pay)
+                                       : run_lanes<double>(ctx, lanes, n_lanes, pays, n_pay);
+    if (st != QK_OK) return st;
+    QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats) * n_lanes * n_pay, hipMemcpyDeviceToHost, ctx->stream));
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    return finish_timing(ctx, lanes[0].n_paths * (int64_t)lanes[0].n_steps * n_lanes, false);
+}
+
+extern "C" qk_status qk_price_payoffs(qk_ctx *ctx, const qk_contract *c, const qk_payoff *pays, int32_t n_pay,
+                                      qk_stats *out) {
+    return qk_price_lanes(ctx, c, 1, pays, n_pay, out);
+}
+
+extern "C" qk_status qk_price(qk_ctx *ctx, const qk_contract *c, const qk_payoff *p, qk_stats *out) {
+    return qk_price_lanes(ctx, c, 1, p, 1, out);
+}
+
+// ---------------------------------------------------------------------------
+// Greeks: the reference's 12-pricing stencil (MCEngine.hpp:576-735) as lanes
+// of one launch under common random numbers, differenced in FP64 on the device.
+// ---------------------------------------------------------------------------
+extern "C" qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk_payoff *pays, int32_t n_pay,
+                                     const qk_bumps *b, qk_greeks *out) {
+    if (!ctx || !out || !b) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_contract(ctx, c);
+    if (st != QK_OK) return st;
+    const double vol0 = std::sqrt(c->v0);
+    if ((b->delta_gamma || b->vanna) && !(b->rel_S > 0.0 && b->rel_S <= 0.05))
+        return fail(ctx, QK_EINVAL, "rel_S must be in (0, 0.05]");
+    if ((b->vega_vomma || b->vanna) && !(b->abs_vol > 0.0 && b->abs_vol < vol0))
+        return fail(ctx, QK_EINVAL, "abs_vol must be in (0, sqrt(v0))");
+    if (b->rho && !(b->abs_r > 0.0 && b->abs_r <= 0.05)) return fail(ctx, QK_EINVAL, "abs_r must be in (0, 0.05]");
+    if (b->theta && !(b->abs_T > 0.0 && b->abs_T < c->T)) return fail(ctx, QK_EINVAL, "abs_T must be in (0, T)");
+    std::vector<qk_contract> L;
+    GreekMap m{};
+    m.base = 0;
+    m.s_up = m.s_dn = m.v_up = m.v_dn = m.r_up = m.r_dn = m.t_dn = m.t_up = m.uu = m.du = m.ud = m.dd = -1;
+    L.push_back(*c);
+    const double h_S = c->S0 * b->rel_S;
+    m.h_S_real = c->price_scale * c->S0 * b->rel_S;
+    m.h_vol = b->abs_vol;
+    m.h_r = b->abs_r;
+    m.h_T = b->abs_T;
+    const double vu = (vol0 + b->abs_vol) * (vol0 + b->abs_vol), vd = (vol0 - b->abs_vol) * (vol0 - b->abs_vol);
+    auto add = [&](auto f) {
+        qk_contract x = *c;
+        f(x);
+        L.push_back(x);
+        return (int32_t)L.size() - 1;
+    };
+    if (b->delta_gamma) {
+        m.s_up = add([&](qk_contract &x) { x.S0 += h_S; });
+        m.s_dn = add([&](qk_contract &x) { x.S0 -= h_S; });
+    }
+    if (b->vega_vomma) {
+        m.v_up = add([&](qk_contract &x) { x.v0 = vu; });
+        m.v_dn = add([&](qk_contract &x) { x.v0 = vd; });
+    }
+    if (b->rho) {
+        m.r_up = add([&](qk_contract &x) { x.r += b->abs_r; });
+        m.r_dn = add([&](qk_contract &x) { x.r -= b->abs_r; });
+    }
+    if (b->theta) {
+        m.t_dn = add([&](qk_contract &x) { x.T -= b->abs_T; });
+        if (b->theta_central) m.t_up = add([&](qk_contract &x) { x.T += b->abs_T; });
+    }
+    if (b->vanna) {
+        m.uu = add([&](qk_contract &x) { x.S0 += h_S; x.v0 = vu; });
+        m.du = add([&](qk_contract &x) { x.S0 -= h_S; x.v0 = vu; });
+        m.ud = add([&](qk_contract &x) { x.S0 += h_S; x.v0 = vd; });
+        m.dd = add([&](qk_contract &x) { x.S0 -= h_S; x.v0 = vd; });
+    }
+    m.n_lanes = (int32_t)L.size();
+    st = validate_lanes(ctx, L.data(), (int32_t)L.size(), pays, n_pay);
+    if (st != QK_OK) return st;
+    if ((st = check_budget(ctx, footprint(*c, (int)L.size(), n_pay, false))) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    st = c->precision == QK_FP32 ? run_lanes<float>(ctx, L.data(), (int)L.size(), pays, n_pay)
+                                 : run_lanes<double>(ctx, L.data(), (int)L.size(), pays, n_pay);
+    if (st != QK_OK) return st;
+    if ((st = ctx->greeks.reserve(ctx, sizeof(qk_greeks) * n_pay)) != QK_OK) return st;
+    QK_HIP(launch_greeks((const qk_stats *)ctx->stats.p, n_pay, m, (qk_greeks *)ctx->greeks.p, ctx->stream));
+    QK_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    QK_HIP(hipMemcpyAsync(out, ctx->greeks.p, sizeof(qk_greeks) * n_pay, hipMemcpyDeviceToHost, ctx->stream));
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps * (int64_t)L.size(), false);
+}
+
+// ---------------------------------------------------------------------------
+// Path store + Longstaff-Schwartz.
+// ---------------------------------------------------------------------------
+namespace qk {
+
+template <typename Real>
+static qk_status generate_store(qk_ctx *ctx, const qk_contract &c) {
+    const LaneC<Real> hl = make_lane<Real>(c);
+    qk_status st;
+    if ((st = ctx->lanes.reserve(ctx, sizeof(LaneC<Real>))) != QK_OK) return st;
+    if ((st = ctx->store.reserve(ctx, (size_t)c.n_paths * c.n_steps * sizeof(Real))) != QK_OK) return st;
+    QK_HIP(hipMemcpyAsync(ctx->lanes.p, &hl, sizeof(hl), hipMemcpyHostToDevice, ctx->stream));
+    PathArgs<Real> a{};
+    a.lanes = (const LaneC<Real> *)ctx->lanes.p;
+    a.pays = nullptr;
+    a.n_lanes = 1;
+    a.n_pay = 0;
+    a.n_steps = c.n_steps;
+    a.n_paths = c.n_paths;
+    a.n_tiles = (c.n_paths + TILE - 1) / TILE;
+    a.key0 = (uint32_t)c.seed;
+    a.key1 = (uint32_t)(c.seed >> 32);
+    a.store = (Real *)ctx->store.p;
+    QK_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    QK_HIP(launch_paths<Real>(a, c.model, c.scheme, true, true, 1, ctx->stream));
+    QK_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    return QK_OK;
+}
+
+template <typename Real>
+static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, double K, qk_stats *out,
+                              double *coeffs_out) {
+    qk_status st = generate_store<Real>(ctx, c);
+    if (st != QK_OK) return st;
+    const int64_t n = c.n_paths;
+    const int N = c.n_steps;
+    const int64_t n_tiles = (n + TILE - 1) / TILE;
+    const int64_t n_blocks = (n + 255) / 256;
+    if ((st = ctx->cf.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
+    if ((st = ctx->coeffs.reserve(ctx, (size_t)N * 4 * 8)) != QK_OK) return st;
+    if ((st = ctx->bpart.reserve(ctx, (size_t)n_blocks * 16 * 8)) != QK_OK) return st;
+    if ((st = ctx->psum.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
+    if ((st = ctx->pm2.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
+    if ((st = ctx->pitm.reserve(ctx, n_tiles * 4)) != QK_OK) return st;
+    if ((st = ctx->disc.reserve(ctx, 8)) != QK_OK) return st;
+    if ((st = ctx->scale.reserve(ctx, 8)) != QK_OK) return st;
+    if ((st = ctx->stats.reserve(ctx, sizeof(qk_stats))) != QK_OK) return st;
+    hipStream_t s = ctx->stream;
+    const double one = 1.0, scale = c.price_scale;
+    QK_HIP(hipMemcpyAsync(ctx->disc.p, &one, 8, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(ctx->scale.p, &scale, 8, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemsetAsync(ctx->coeffs.p, 0, (size_t)N * 4 * 8, s));
+    LsmArgs<Real> a{};
+    a.store = (const Real *)ctx->store.p;
+    a.cf = (Real *)ctx->cf.p;
+    a.coeffs = (double *)ctx->coeffs.p;
+    a.bpart = (double *)ctx->bpart.p;
+    a.sums_out = nullptr;
+    a.counter = (unsigned int *)ctx->counter.p;
+    a.n_paths = n;
+    a.n_steps = N;
+    a.right = right;
+    a.K = (Real)K;
+    a.disc = (Real)std::exp(-c.r * (c.T / N));          // ForwardBackwardMC.hpp:209
+    for (int t = N - 1; t >= 1; --t) {                  // ForwardBackwardMC.hpp:219-242
+        a.t_upd = t;
+        a.t_gram = t - 1;
+        QK_HIP(launch_lsm_sweep<Real>(a, s));
+    }
+    a.t_upd = 0;
+    a.t_gram = -1;
+    QK_HIP(launch_lsm_final<Real>(a, (double *)ctx->psum.p, (double *)ctx->pm2.p, (uint32_t *)ctx->pitm.p, n_tiles,
+                                  (Real)(c.precision == QK_FP32 ? 1e-6 : 1e-12), s));
+    QK_HIP(hipEventRecord(ctx->ev[3], s));
+    QK_HIP(launch_merge((const double *)ctx->psum.p, (const double *)ctx->pm2.p, (const uint32_t *)ctx->pitm.p,
+                        n_tiles, n, 1, 1, (const double *)ctx->disc.p, (const double *)ctx->scale.p,
+                        (qk_stats *)ctx->stats.p, s));
+    QK_HIP(hipEventRecord(ctx->ev[2], s));
+    QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, s));
+    if (coeffs_out) QK_HIP(hipMemcpyAsync(coeffs_out, ctx->coeffs.p, (size_t)N * 4 * 8, hipMemcpyDeviceToHost, s));
+    QK_HIP(hipStreamSynchronize(s));
+    // Immediate exercise check (LSMAmerican.hpp:409-418): if today's intrinsic
+    // beats the mean discounted cash flow, every path exercises at t0.
+    const Real S0 = (Real)c.S0, Kr = (Real)K;
+    const Real today = right == QK_CALL ? (S0 > Kr ? S0 - Kr : Real(0)) : (Kr > S0 ? Kr - S0 : Real(0));
+    if ((double)today > out->mean) {
+        const double eps = c.precision == QK_FP32 ? 1e-6 : 1e-12;
+        out->mean = (double)today;
+        out->m2 = 0.0;
+        out->price = (double)today * c.price_scale;
+        out->se = 0.0;
+        out->itm = (double)today > eps ? n : 0;
+        out->prob_itm = (double)out->itm / (double)n;
+    }
+    return QK_OK;
+}
+
+}  // namespace qk
+
+extern "C" qk_status qk_price_american(qk_ctx *ctx, const qk_contract *c, int32_t right, double K, qk_stats *out,
+                                       double *coeffs_out) {
+    if (!ctx || !out) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_contract(ctx, c);
+    if (st != QK_OK) return st;
+    if (right != QK_CALL && right != QK_PUT) return fail(ctx, QK_EINVAL, "right out of range");
+    if (!(K > 0.0)) return fail(ctx, QK_EINVAL, "K must be positive");
+    if ((st = check_budget(ctx, footprint(*c, 1, 1, true))) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    st = c->precision == QK_FP32 ? run_american<float>(ctx, *c, right, K, out, coeffs_out)
+                                 : run_american<double>(ctx, *c, right, K, out, coeffs_out);
+    if (st != QK_OK) return st;
+    return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, true);
+}
+
+extern "C" qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, double *S_out) {
+    if (!ctx || !S_out) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_contract(ctx, c);
+    if (st != QK_OK) return st;
+    if ((st = check_budget(ctx, footprint(*c, 1, 1, true))) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    const size_t n = (size_t)c->n_paths * c->n_steps;
+    if (c->precision == QK_FP64) {
+        if ((st = generate_store<double>(ctx, *c)) != QK_OK) return st;
+        QK_HIP(hipMemcpyAsync(S_out, ctx->store.p, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        QK_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        if ((st = generate_store<float>(ctx, *c)) != QK_OK) return st;
+        std::vector<float> tmp(n);
+        QK_HIP(hipMemcpyAsync(tmp.data(), ctx->store.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        QK_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < n; ++i) S_out[i] = tmp[i];
+    }
+    QK_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, false);
+}

@@ -1,0 +1,42 @@
+// qk_capi_internal.hpp -- the qk_ctx definition and helpers shared by the C ABI
+// translation units (pricing entry points and the request server).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "qk_types.hpp"
+
+namespace qk {
+
+// A device buffer that only grows; freed with its context.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    qk_status reserve(qk_ctx *ctx, size_t bytes);
+    void release();
+};
+
+qk_status fail(qk_ctx *ctx, qk_status s, const std::string &msg);
+qk_status hip_fail(qk_ctx *ctx, hipError_t e, const char *what);
+qk_status validate_contract(qk_ctx *ctx, const qk_contract *c);
+qk_status validate_payoff(qk_ctx *ctx, const qk_payoff *p);
+bool path_dependent(int type);
+template <typename Real>
+LaneC<Real> make_lane(const qk_contract &c);
+size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american);
+qk_status check_budget(qk_ctx *ctx, size_t bytes);
+
+}  // namespace qk
+
+struct qk_ctx {
+    int device = 0;
+    size_t budget = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::string err;
+    qk_timing timing{};
+    qk::DevBuf lanes, pays, psum, pm2, pitm, disc, scale, stats, greeks, store, cf, coeffs, bpart, counter, sums;
+    ~qk_ctx();
+};

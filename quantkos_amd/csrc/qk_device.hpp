@@ -1,0 +1,226 @@
+// qk_device.hpp -- device-side building blocks of the fused path kernels
+// (gfx950, wave64): counter-based RNG, per-precision math, Heston/Bates steps,
+// payoff evaluation and wavefront reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qk_types.hpp"
+
+namespace qk {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  The key is uniform across a launch,
+// so the key schedule stays in scalar registers; each round is two 32x32->64
+// multiplies and two XORs per output pair.
+// ---------------------------------------------------------------------------
+struct Words { uint32_t w0, w1, w2, w3; };
+
+__device__ __forceinline__ Words philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                        uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return Words{c0, c1, c2, c3};
+}
+
+// ---------------------------------------------------------------------------
+// Per-precision math.  FP32 uses the CDNA transcendental unit directly
+// (v_exp_f32 / v_log_f32 are base-2, v_sin_f32 / v_cos_f32 take revolutions);
+// FP64 uses the IEEE library routines.
+// ---------------------------------------------------------------------------
+template <typename Real> struct Math;
+
+template <> struct Math<float> {
+    __device__ static __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+    __device__ static __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+    __device__ static __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    __device__ static __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static __forceinline__ float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+    __device__ static __forceinline__ float fmin(float a, float b) { return __builtin_fminf(a, b); }
+    // sin/cos of 2*pi*u for u in (0, 1).
+    __device__ static __forceinline__ void sincos2pi(float u, float &s, float &c) {
+        s = __builtin_amdgcn_sinf(u);
+        c = __builtin_amdgcn_cosf(u);
+    }
+    // Top 24 bits of a Philox word, centred: ((w >> 8) + 0.5) * 2^-24, exact in FP32.
+    __device__ static __forceinline__ float uniform(uint32_t w) {
+        return __builtin_fmaf((float)(w >> 8), 0x1.0p-24f, 0x1.0p-25f);
+    }
+};
+
+template <> struct Math<double> {
+    __device__ static __forceinline__ double exp(double x) { return ::exp(x); }
+    __device__ static __forceinline__ double log(double x) { return ::log(x); }
+    __device__ static __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
+    __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+    __device__ static __forceinline__ double fmax(double a, double b) { return ::fmax(a, b); }
+    __device__ static __forceinline__ double fmin(double a, double b) { return ::fmin(a, b); }
+    __device__ static __forceinline__ void sincos2pi(double u, double &s, double &c) {
+        ::sincospi(2.0 * u, &s, &c);
+    }
+    // All 32 bits of a Philox word, centred: (w + 0.5) * 2^-32, exact in FP64.
+    __device__ static __forceinline__ double uniform(uint32_t w) {
+        return __builtin_fma((double)w, 0x1.0p-32, 0x1.0p-33);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// One Heston step on (S or log S, v) given the slot-0 words of (path, step).
+// Formulas follow MSchemes.hpp: Euler :133-155, implicit Milstein :229-252,
+// Andersen QE :363-441.  Returns the log-increment of S.
+// ---------------------------------------------------------------------------
+template <typename Real, int SCHEME>
+__device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real z2, uint32_t w2, Real &v) {
+    using M = Math<Real>;
+    const Real v_n = v;
+    if constexpr (SCHEME == SCHEME_EULER || SCHEME == SCHEME_MILSTEIN) {
+        const Real zs = c.rho * z1 + c.rho_c * z2;
+        const Real sv = M::sqrt(v_n);
+        const Real expo = c.rqdt - v_n * c.half_dt + (sv * c.sqrt_dt) * zs;
+        Real vn;
+        if constexpr (SCHEME == SCHEME_EULER) {
+            vn = v_n * c.omkdt + c.ktdt + (c.ssdt * sv) * z1;
+        } else {
+            vn = c.iden * (v_n + c.ktdt + (c.ssdt * sv) * z1 + c.qs2dt * (z1 * z1 - Real(1)));
+        }
+        v = M::fmax(vn, Real(0));
+        return expo;
+    } else {
+        // Andersen QE with martingale correction.  The exponential branch draws
+        // its uniform from word 2 of the block (same law as Phi(Z_V)).
+        const Real m = v_n * c.ekdt + c.tmf;
+        const Real s2 = v_n * c.s2f1 + c.s2f2;
+        Real vn, logM;
+        if (m <= c.eps) {
+            vn = Real(0);
+            logM = Real(0);
+        } else {
+            const Real rm = M::rcp(m);
+            const Real psi = s2 * rm * rm;
+            if (psi <= c.eps) {
+                vn = m;
+                logM = Real(0);
+            } else if (psi <= Real(1.5)) {
+                const Real two_ip = Real(2) * M::rcp(psi);
+                const Real b2 = two_ip - Real(1) + M::sqrt(two_ip * (two_ip - Real(1)));
+                const Real a = M::div(m, Real(1) + b2);
+                const Real bz = M::sqrt(b2) + z1;
+                vn = a * bz * bz;
+                const Real safe_A = M::fmin(c.A, Real(0.5) * (Real(1) + b2) * rm - c.eps);
+                const Real tAa = Real(1) - Real(2) * safe_A * a;
+                // log M = A b^2 a / (1 - 2 A a) - log(1 - 2 A a) / 2
+                logM = M::div(safe_A * b2 * a, tAa) - Real(0.5) * M::log(tAa);
+            } else {
+                const Real p = M::div(psi - Real(1), psi + Real(1));
+                const Real beta = (Real(1) - p) * rm;
+                const Real U = M::uniform(w2);
+                if (U <= p) {
+                    vn = Real(0);
+                } else {
+                    const Real safe_U = M::fmin(U, Real(1) - c.eps);
+                    vn = M::div(M::log(M::div(Real(1) - p, Real(1) - safe_U)), beta);
+                }
+                const Real safe_A = M::fmin(c.A, beta - c.eps);
+                logM = M::log(p + M::div(beta * (Real(1) - p), beta - safe_A));
+            }
+        }
+        v = vn;
+        // K0 + K1 v = -log M - (K3/2) v  (MSchemes.hpp:434-439)
+        return c.rqdt - logM + c.cv * v_n + c.K2 * vn + M::sqrt(c.K3 * v_n + c.K4 * vn) * z2;
+    }
+}
+
+// Extra Bates uniform e_j of (path, step): word (j mod 4) of block slot 1 + j/4.
+template <typename Real>
+__device__ __forceinline__ Real bates_extra(uint32_t p0, uint32_t p1, uint32_t t, uint32_t j,
+                                            uint32_t k0, uint32_t k1) {
+    const Words w = philox(p0, p1, t, 1u + (j >> 2), k0, k1);
+    const uint32_t x = (j & 3u) == 0 ? w.w0 : (j & 3u) == 1 ? w.w1 : (j & 3u) == 2 ? w.w2 : w.w3;
+    return Math<Real>::uniform(x);
+}
+
+// Compound-Poisson log-normal jump of one step (MSchemes.hpp:493-527, Knuth's
+// product of uniforms).  Returns the log jump (0 when no jump).
+template <typename Real>
+__device__ __noinline__ Real bates_jump_slow(const LaneC<Real> &c, Real u3, uint32_t p0, uint32_t p1,
+                                             uint32_t t, uint32_t k0, uint32_t k1) {
+    using M = Math<Real>;
+    int N = 0;
+    Real p = u3;
+    uint32_t j = 0;
+    while (p >= c.eml_dt && N < 64) {
+        ++N;
+        p *= bates_extra<Real>(p0, p1, t, j++, k0, k1);
+    }
+    const Real e0 = bates_extra<Real>(p0, p1, t, j, k0, k1);
+    const Real e1 = bates_extra<Real>(p0, p1, t, j + 1, k0, k1);
+    Real s, co;
+    M::sincos2pi(e1, s, co);
+    const Real z = M::sqrt(Real(-2) * M::log(e0)) * co;
+    const Real n = Real(N);
+    return n * c.mu_j + M::sqrt(n) * c.sigma_j * z;
+}
+
+// Vanilla payoff (Payoff.hpp:41-72).
+template <typename Real>
+__device__ __forceinline__ Real vanilla(int right, Real S, Real K) {
+    return right == QK_CALL ? (S > K ? S - K : Real(0)) : (K > S ? K - S : Real(0));
+}
+
+// evaluate_final_payoff (Payoff.hpp:157-242).  sum excludes S0; max/min start at S0.
+template <typename Real>
+__device__ __forceinline__ Real payoff_eval(const PayD<Real> &p, Real S, double sum, Real mx, Real mn,
+                                            int n_steps) {
+    const Real K = p.K, B = p.B;
+    const bool call = p.right == QK_CALL;
+    switch (p.type) {
+    case QK_EUROPEAN: return vanilla<Real>(p.right, S, K);
+    case QK_ASIAN: return vanilla<Real>(p.right, Real(sum / (double)n_steps), K);
+    case QK_BARRIER_UP_OUT: return mx < B ? vanilla<Real>(p.right, S, K) : Real(0);
+    case QK_BARRIER_UP_IN: return mx >= B ? vanilla<Real>(p.right, S, K) : Real(0);
+    case QK_BARRIER_DOWN_OUT: return mn > B ? vanilla<Real>(p.right, S, K) : Real(0);
+    case QK_BARRIER_DOWN_IN: return mn <= B ? vanilla<Real>(p.right, S, K) : Real(0);
+    case QK_LOOKBACK_FIXED: return call ? vanilla<Real>(QK_CALL, mx, K) : vanilla<Real>(QK_PUT, mn, K);
+    case QK_LOOKBACK_FLOATING: return call ? vanilla<Real>(QK_CALL, S, mn) : vanilla<Real>(QK_PUT, S, mx);
+    case QK_BINARY_CASH: return call ? (S > K ? Real(1) : Real(0)) : (S < K ? Real(1) : Real(0));
+    case QK_BINARY_ASSET: return call ? (S > K ? S : Real(0)) : (S < K ? S : Real(0));
+    default: return Real(0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront (64-lane) reductions.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// Chan et al. pairwise merge of (n, mean, M2) moments.
+__device__ __forceinline__ void chan_merge(double &n_a, double &mean_a, double &m2_a, double n_b,
+                                           double mean_b, double m2_b) {
+    if (n_b <= 0.0) return;
+    if (n_a <= 0.0) {
+        n_a = n_b; mean_a = mean_b; m2_a = m2_b;
+        return;
+    }
+    const double n = n_a + n_b;
+    const double delta = mean_b - mean_a;
+    mean_a = mean_a + delta * (n_b / n);
+    m2_a = m2_a + m2_b + delta * delta * (n_a * n_b / n);
+    n_a = n;
+}
+
+}  // namespace qk

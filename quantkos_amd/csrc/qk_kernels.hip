@@ -1,0 +1,432 @@
+// qk_kernels.hip -- hand-written gfx950 kernels of the Monte Carlo hot path.
+//
+//  k_paths      fused path generation: Philox normals -> Heston/Bates step ->
+//               payoff tracking -> per-tile (64-path) moments of every payoff.
+//               Replaces MarkovianIntegrationKernel (MSolver.hpp:98-130) and the
+//               two reductions of OptionPricer::accumulate_batch_payoffs
+//               (OptionPricer.hpp:129-146).  L lanes (bumped contracts) per
+//               thread share one Philox block per step (common random numbers).
+//  k_merge      fixed-order Chan merge of tile moments + FP64 finalisation
+//               (OptionPricer.hpp:152-172, :204-249).
+//  k_greeks     FP64 central differences of lane prices (MCEngine.hpp:586-730).
+//  k_lsm_sweep  one Longstaff-Schwartz date: cash-flow update at date t and the
+//               Laguerre Gram / right-hand side at date t-1, block partials, and
+//               the last block's FP64 Cholesky solve (LSMAmerican.hpp:310-341,
+//               :433-473, :516-613).
+//  k_lsm_final  date-0 update, final discount and tile moments (:393-419).
+#include <hip/hip_runtime.h>
+
+#include "qk_device.hpp"
+#include "qk_launch.hpp"
+
+namespace qk {
+
+// ---------------------------------------------------------------------------
+// Fused path kernel.  One wavefront = one tile of 64 consecutive paths, one
+// path per lane; blockIdx.y selects a group of L lanes.  Tile moments are
+// stored per (lane, payoff, tile) so the merge order is fixed by the tile
+// index: results do not depend on the grid shape or on other work in flight.
+// ---------------------------------------------------------------------------
+template <typename Real, int MODEL, int SCHEME, bool PATHDEP, bool STORE, int L>
+__global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
+    using M = Math<Real>;
+    const int lid = threadIdx.x & (TILE - 1);
+    const int64_t tile = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    if (tile >= a.n_tiles) return;                       // uniform per wavefront
+    const int64_t path = tile * TILE + lid;
+    const bool active = path < a.n_paths;
+    const uint32_t p0 = (uint32_t)(uint64_t)path, p1 = (uint32_t)((uint64_t)path >> 32);
+    const int lbase = blockIdx.y * L;
+    const LaneC<Real> *lc = a.lanes + lbase;
+
+    Real S[L], v[L], mx[L], mn[L];
+    double sm[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        S[l] = PATHDEP ? lc[l].S0 : Real(0);     // log-increment accumulator when !PATHDEP
+        v[l] = lc[l].v0;
+        mx[l] = mn[l] = lc[l].S0;
+        sm[l] = 0.0;
+    }
+
+    for (int t = 0; t < a.n_steps; ++t) {
+        const Words w = philox(p0, p1, (uint32_t)t, 0u, a.key0, a.key1);
+        const Real u0 = M::uniform(w.w0), u1 = M::uniform(w.w1);
+        Real sn, cs;
+        M::sincos2pi(u1, sn, cs);
+        const Real R = M::sqrt(Real(-2) * M::log(u0));
+        const Real z1 = R * cs, z2 = R * sn;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            Real inc = heston_step<Real, SCHEME>(lc[l], z1, z2, w.w2, v[l]);
+            if constexpr (MODEL == MODEL_BATES) {
+                const Real u3 = M::uniform(w.w3);
+                if (u3 >= lc[l].eml_dt) inc += bates_jump_slow<Real>(lc[l], u3, p0, p1, (uint32_t)t, a.key0, a.key1);
+            }
+            if constexpr (PATHDEP) {
+                S[l] *= M::exp(inc);
+                sm[l] += (double)S[l];
+                mx[l] = M::fmax(mx[l], S[l]);
+                mn[l] = M::fmin(mn[l], S[l]);
+            } else {
+                S[l] += inc;
+            }
+            if constexpr (STORE) {
+                if (active) a.store[(int64_t)t * a.n_paths + path] = S[l];
+            }
+        }
+    }
+
+    const unsigned long long amask = __ballot(active);
+    const double n_act = (double)__popcll(amask);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        if (lbase + l >= a.n_lanes) break;
+        const Real ST = PATHDEP ? S[l] : lc[l].S0 * M::exp(S[l]);
+        for (int j = 0; j < a.n_pay; ++j) {
+            const Real x = active ? payoff_eval<Real>(a.pays[j], ST, sm[l], mx[l], mn[l], a.n_steps) : Real(0);
+            const double xd = (double)x;
+            const double s = wave_sum(xd);
+            const double mean = s / n_act;
+            const double d = active ? xd - mean : 0.0;
+            const double m2 = wave_sum(d * d);
+            const unsigned long long im = __ballot(active && x > a.itm_eps);
+            if (lid == 0) {
+                const int64_t idx = ((int64_t)(lbase + l) * a.n_pay + j) * a.n_tiles + tile;
+                a.part_sum[idx] = s;
+                a.part_m2[idx] = m2;
+                a.part_itm[idx] = (uint32_t)__popcll(im);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-order merge of tile moments and FP64 finalisation.  One block per
+// (lane, payoff); thread i merges tiles i, i+256, ... then a fixed LDS tree.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_merge(const double *__restrict__ ps, const double *__restrict__ pm2,
+                                               const uint32_t *__restrict__ pitm, int64_t n_tiles, int64_t n_paths,
+                                               int n_pay, const double *__restrict__ disc,
+                                               const double *__restrict__ scale, qk_stats *__restrict__ out) {
+    const int g = blockIdx.x;
+    const int lane = g / n_pay, j = g % n_pay;
+    const int64_t base = (int64_t)g * n_tiles;
+    double n = 0.0, mean = 0.0, m2 = 0.0;
+    unsigned long long itm = 0;
+    for (int64_t t = threadIdx.x; t < n_tiles; t += 256) {
+        const int64_t rem = n_paths - t * TILE;
+        const double nt = (double)(rem < TILE ? rem : TILE);
+        chan_merge(n, mean, m2, nt, ps[base + t] / nt, pm2[base + t]);
+        itm += pitm[base + t];
+    }
+    __shared__ double sn[256], smu[256], sq[256];
+    __shared__ unsigned long long si[256];
+    sn[threadIdx.x] = n; smu[threadIdx.x] = mean; sq[threadIdx.x] = m2; si[threadIdx.x] = itm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            double na = sn[threadIdx.x], ma = smu[threadIdx.x], qa = sq[threadIdx.x];
+            chan_merge(na, ma, qa, sn[threadIdx.x + s], smu[threadIdx.x + s], sq[threadIdx.x + s]);
+            sn[threadIdx.x] = na; smu[threadIdx.x] = ma; sq[threadIdx.x] = qa;
+            si[threadIdx.x] += si[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double N = sn[0];
+        const double var = N > 1.0 ? sq[0] / (N - 1.0) : 0.0;
+        const double f = disc[lane] * scale[j];
+        qk_stats o;
+        o.price = smu[0] * f;
+        o.se = sqrt(var / N) * f;
+        o.prob_itm = (double)si[0] / N;
+        o.mean = smu[0];
+        o.m2 = sq[0];
+        o.n = (int64_t)N;
+        o.itm = (int64_t)si[0];
+        out[g] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Greeks: FP64 stencils of MCEngine.hpp:586-730 over lane prices.
+// ---------------------------------------------------------------------------
+__global__ void k_greeks(const qk_stats *__restrict__ st, int n_pay, GreekMap m, qk_greeks *__restrict__ out) {
+    const int j = threadIdx.x;
+    if (j >= n_pay) return;
+    auto P = [&](int lane) { return st[lane * n_pay + j].price; };
+    qk_greeks g = {};
+    const double P0 = P(m.base);
+    g.base_price = P0;
+    g.base_se = st[m.base * n_pay + j].se;
+    g.n_lanes = m.n_lanes;
+    if (m.s_up >= 0) {
+        const double pu = P(m.s_up), pd = P(m.s_dn);
+        g.delta = (pu - pd) / (2.0 * m.h_S_real);
+        g.gamma = (pu - 2.0 * P0 + pd) / (m.h_S_real * m.h_S_real);
+    }
+    if (m.v_up >= 0) {
+        const double pu = P(m.v_up), pd = P(m.v_dn);
+        g.vega = (pu - pd) / (2.0 * m.h_vol);
+        g.vomma = (pu - 2.0 * P0 + pd) / (m.h_vol * m.h_vol);
+    }
+    if (m.r_up >= 0) g.rho = (P(m.r_up) - P(m.r_dn)) / (2.0 * m.h_r);
+    if (m.t_dn >= 0) {
+        if (m.t_up >= 0) g.theta = (P(m.t_dn) - P(m.t_up)) / (2.0 * m.h_T);
+        else g.theta = (P(m.t_dn) - P0) / m.h_T;
+    }
+    if (m.uu >= 0) g.vanna = (P(m.uu) - P(m.du) - P(m.ud) + P(m.dd)) / (4.0 * m.h_S_real * m.h_vol);
+    out[j] = g;
+}
+
+// ---------------------------------------------------------------------------
+// Longstaff-Schwartz.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__device__ __forceinline__ void laguerre4(Real X, Real L[4]) {
+    // Unweighted Laguerre polynomials, Horner form (LSMAmerican.hpp:172-198).
+    L[0] = Real(1);
+    L[1] = Real(1) - X;
+    L[2] = ((Real(0.5) * X - Real(2)) * X) + Real(1);
+    L[3] = (((Real(-1.0 / 6.0) * X + Real(1.5)) * X - Real(3)) * X) + Real(1);
+}
+
+// Cholesky solve with the reference's 1e-8 ridge, 1e-14 pivot limit and
+// mean-payoff fallback (LSMAmerican.hpp:516-581, :605-613).  s = {count, M00,
+// M01, M02, M03, M11, M12, M13, M22, M23, M33, B0..B3}.
+__device__ void lsm_solve(const double s[15], double C[4]) {
+    const double count = s[0];
+    C[0] = C[1] = C[2] = C[3] = 0.0;
+    if (count < 4.0) {
+        if (count > 0.0) C[0] = s[11] / count;
+        return;
+    }
+    const int up[4][4] = {{1, 2, 3, 4}, {2, 5, 6, 7}, {3, 6, 8, 9}, {4, 7, 9, 10}};
+    double Lm[4][4] = {};
+    for (int i = 0; i < 4; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < j; ++k) acc += Lm[i][k] * Lm[j][k];
+            const double Mij = s[up[i][j]] + (i == j ? 1e-8 : 0.0);
+            if (i == j) {
+                const double val = Mij - acc;
+                if (val < 1e-14) {
+                    C[0] = s[11] / count;
+                    C[1] = C[2] = C[3] = 0.0;
+                    return;
+                }
+                Lm[i][i] = sqrt(val);
+            } else {
+                Lm[i][j] = (Mij - acc) / Lm[j][j];
+            }
+        }
+    }
+    double y[4];
+    for (int i = 0; i < 4; ++i) {
+        double acc = 0.0;
+        for (int k = 0; k < i; ++k) acc += Lm[i][k] * y[k];
+        y[i] = (s[11 + i] - acc) / Lm[i][i];
+    }
+    for (int i = 3; i >= 0; --i) {
+        double acc = 0.0;
+        for (int k = i + 1; k < 4; ++k) acc += Lm[k][i] * C[k];
+        C[i] = (y[i] - acc) / Lm[i][i];
+    }
+}
+
+// New cash flow of one path at date t (initialisation when t == n_steps-1).
+template <typename Real>
+__device__ __forceinline__ Real lsm_cashflow(const LsmArgs<Real> &a, int64_t i, int t, Real cf_old) {
+    const Real S = a.store[(int64_t)t * a.n_paths + i];
+    const Real intrinsic = vanilla<Real>(a.right, S, a.K);
+    if (t == a.n_steps - 1) return intrinsic;
+    const Real dcf = cf_old * a.disc;
+    if (intrinsic > Real(0)) {
+        Real L[4];
+        laguerre4<Real>(Math<Real>::div(S, a.K), L);
+        const double *C = a.coeffs + (int64_t)t * 4;
+        double cont = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cont += C[k] * (double)L[k];
+        return (double)intrinsic > cont ? intrinsic : dcf;
+    }
+    return dcf;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_lsm_sweep(LsmArgs<Real> a) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double acc[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) acc[k] = 0.0;
+    if (i < a.n_paths) {
+        const Real c = lsm_cashflow<Real>(a, i, a.t_upd, a.t_upd == a.n_steps - 1 ? Real(0) : a.cf[i]);
+        a.cf[i] = c;
+        const Real S = a.store[(int64_t)a.t_gram * a.n_paths + i];
+        if (vanilla<Real>(a.right, S, a.K) > Real(0)) {
+            Real Lr[4];
+            laguerre4<Real>(Math<Real>::div(S, a.K), Lr);
+            const double Y = (double)(c * a.disc);
+            double L[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) L[k] = (double)Lr[k];
+            acc[0] = 1.0;
+            acc[1] = L[0] * L[0]; acc[2] = L[0] * L[1]; acc[3] = L[0] * L[2]; acc[4] = L[0] * L[3];
+            acc[5] = L[1] * L[1]; acc[6] = L[1] * L[2]; acc[7] = L[1] * L[3];
+            acc[8] = L[2] * L[2]; acc[9] = L[2] * L[3]; acc[10] = L[3] * L[3];
+            acc[11] = L[0] * Y; acc[12] = L[1] * Y; acc[13] = L[2] * Y; acc[14] = L[3] * Y;
+        }
+    }
+    // Block partial: wavefront sums, then the 4 wavefronts in order.
+    __shared__ double wpart[WAVES_PER_BLOCK][16];
+    const int lid = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 15; ++k) {
+        const double s = wave_sum(acc[k]);
+        if (lid == 0) wpart[wv][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 15) {
+        double s = 0.0;
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) s += wpart[w][threadIdx.x];
+        a.bpart[(int64_t)blockIdx.x * 16 + threadIdx.x] = s;
+    }
+    // Last block to finish reduces all block partials in block order and solves.
+    __threadfence();
+    __shared__ int is_last;
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double r[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) r[k] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) {
+#pragma unroll
+        for (int k = 0; k < 15; ++k) r[k] += a.bpart[(int64_t)b * 16 + k];
+    }
+    __shared__ double red[15][256];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) red[k][threadIdx.x] = r[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int k = 0; k < 15; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double s[15], C[4];
+        for (int k = 0; k < 15; ++k) s[k] = red[k][0];
+        lsm_solve(s, C);
+        for (int k = 0; k < 4; ++k) a.coeffs[(int64_t)a.t_gram * 4 + k] = C[k];
+        if (a.sums_out) {
+            for (int k = 0; k < 15; ++k) a.sums_out[(int64_t)a.t_gram * 16 + k] = s[k];
+        }
+        *a.counter = 0u;
+    }
+}
+
+// Date-0 update (or the initialisation when n_steps == 1), final one-step
+// discount, and the tile moments of the discounted cash flows.
+template <typename Real>
+__global__ __launch_bounds__(BLOCK) void k_lsm_final(LsmArgs<Real> a, double *part_sum, double *part_m2,
+                                                      uint32_t *part_itm, int64_t n_tiles, Real itm_eps) {
+    const int lid = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    if (tile >= n_tiles) return;
+    const int64_t i = tile * TILE + lid;
+    const bool active = i < a.n_paths;
+    Real x = Real(0);
+    if (active) {
+        const Real c = lsm_cashflow<Real>(a, i, a.t_upd, a.t_upd == a.n_steps - 1 ? Real(0) : a.cf[i]);
+        x = c * a.disc;
+        a.cf[i] = x;
+    }
+    const double n_act = (double)__popcll(__ballot(active));
+    const double xd = (double)x;
+    const double s = wave_sum(xd);
+    const double mean = s / n_act;
+    const double d = active ? xd - mean : 0.0;
+    const double m2 = wave_sum(d * d);
+    const unsigned long long im = __ballot(active && x > itm_eps);
+    if (lid == 0) {
+        part_sum[tile] = s;
+        part_m2[tile] = m2;
+        part_itm[tile] = (uint32_t)__popcll(im);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+template <typename Real, int MODEL, int SCHEME>
+static void launch_ms(const PathArgs<Real> &a, bool pathdep, bool store, int L, dim3 grid, hipStream_t s) {
+    if (store) {
+        hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, true, 1>), grid, dim3(BLOCK), 0, s, a);
+    } else if (L == 4) {
+        if (pathdep) hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, false, 4>), grid, dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, false, false, 4>), grid, dim3(BLOCK), 0, s, a);
+    } else {
+        if (pathdep) hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, false, 1>), grid, dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, false, false, 1>), grid, dim3(BLOCK), 0, s, a);
+    }
+}
+
+template <typename Real>
+hipError_t launch_paths(const PathArgs<Real> &a, int model, int scheme, bool pathdep, bool store, int L,
+                        hipStream_t s) {
+    const dim3 grid((unsigned)((a.n_tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK), (unsigned)((a.n_lanes + L - 1) / L));
+    switch (model * 3 + scheme) {
+    case 0: launch_ms<Real, MODEL_HESTON, SCHEME_EULER>(a, pathdep, store, L, grid, s); break;
+    case 1: launch_ms<Real, MODEL_HESTON, SCHEME_MILSTEIN>(a, pathdep, store, L, grid, s); break;
+    case 2: launch_ms<Real, MODEL_HESTON, SCHEME_QE>(a, pathdep, store, L, grid, s); break;
+    case 3: launch_ms<Real, MODEL_BATES, SCHEME_EULER>(a, pathdep, store, L, grid, s); break;
+    case 4: launch_ms<Real, MODEL_BATES, SCHEME_MILSTEIN>(a, pathdep, store, L, grid, s); break;
+    case 5: launch_ms<Real, MODEL_BATES, SCHEME_QE>(a, pathdep, store, L, grid, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pitm, int64_t n_tiles, int64_t n_paths,
+                        int n_lanes, int n_pay, const double *disc, const double *scale, qk_stats *out,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_merge, dim3(n_lanes * n_pay), dim3(256), 0, s, ps, pm2, pitm, n_tiles, n_paths, n_pay,
+                       disc, scale, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_greeks, dim3(1), dim3(64), 0, s, st, n_pay, m, out);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_lsm_sweep(const LsmArgs<Real> &a, hipStream_t s) {
+    const unsigned nb = (unsigned)((a.n_paths + 255) / 256);
+    hipLaunchKernelGGL(k_lsm_sweep<Real>, dim3(nb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_lsm_final(const LsmArgs<Real> &a, double *ps, double *pm2, uint32_t *pitm, int64_t n_tiles,
+                            Real itm_eps, hipStream_t s) {
+    const unsigned nb = (unsigned)((n_tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    hipLaunchKernelGGL(k_lsm_final<Real>, dim3(nb), dim3(BLOCK), 0, s, a, ps, pm2, pitm, n_tiles, itm_eps);
+    return hipGetLastError();
+}
+
+template hipError_t launch_paths<float>(const PathArgs<float> &, int, int, bool, bool, int, hipStream_t);
+template hipError_t launch_paths<double>(const PathArgs<double> &, int, int, bool, bool, int, hipStream_t);
+template hipError_t launch_lsm_sweep<float>(const LsmArgs<float> &, hipStream_t);
+template hipError_t launch_lsm_sweep<double>(const LsmArgs<double> &, hipStream_t);
+template hipError_t launch_lsm_final<float>(const LsmArgs<float> &, double *, double *, uint32_t *, int64_t, float,
+                                            hipStream_t);
+template hipError_t launch_lsm_final<double>(const LsmArgs<double> &, double *, double *, uint32_t *, int64_t, double,
+                                             hipStream_t);
+
+}  // namespace qk

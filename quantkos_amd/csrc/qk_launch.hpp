@@ -1,0 +1,44 @@
+// qk_launch.hpp -- kernel launchers (qk_kernels.hip) called by the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qk_types.hpp"
+
+namespace qk {
+
+// Lane indices of the Greek stencil (-1 = not priced) and the real-scale bumps.
+struct GreekMap {
+    int32_t base, s_up, s_dn, v_up, v_dn, r_up, r_dn, t_dn, t_up, uu, du, ud, dd, n_lanes;
+    double h_S_real, h_vol, h_r, h_T;
+};
+
+template <typename Real>
+struct LsmArgs {
+    const Real *store;     // [n_steps][n_paths]
+    Real *cf;              // [n_paths]
+    double *coeffs;        // [n_steps][4]
+    double *bpart;         // [n_blocks][16]
+    double *sums_out;      // [n_steps][16] or null
+    unsigned int *counter; // zero between launches
+    int64_t n_paths;
+    int32_t n_steps, right;
+    int32_t t_upd, t_gram;
+    Real K, disc;
+};
+
+template <typename Real>
+hipError_t launch_paths(const PathArgs<Real> &a, int model, int scheme, bool pathdep, bool store, int L,
+                        hipStream_t s);
+hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pitm, int64_t n_tiles, int64_t n_paths,
+                        int n_lanes, int n_pay, const double *disc, const double *scale, qk_stats *out,
+                        hipStream_t s);
+hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s);
+template <typename Real>
+hipError_t launch_lsm_sweep(const LsmArgs<Real> &a, hipStream_t s);
+template <typename Real>
+hipError_t launch_lsm_final(const LsmArgs<Real> &a, double *ps, double *pm2, uint32_t *pitm, int64_t n_tiles,
+                            Real itm_eps, hipStream_t s);
+
+}  // namespace qk

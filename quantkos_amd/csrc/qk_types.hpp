@@ -1,0 +1,55 @@
+// qk_types.hpp -- structures shared by the host launch code and the kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/qk.h"
+
+namespace qk {
+
+constexpr int SCHEME_EULER = QK_SCHEME_EULER;
+constexpr int SCHEME_MILSTEIN = QK_SCHEME_MILSTEIN;
+constexpr int SCHEME_QE = QK_SCHEME_QE;
+constexpr int MODEL_HESTON = QK_MODEL_HESTON;
+constexpr int MODEL_BATES = QK_MODEL_BATES;
+
+constexpr int TILE = 64;          // paths per tile = one wavefront
+constexpr int WAVES_PER_BLOCK = 4;
+constexpr int BLOCK = TILE * WAVES_PER_BLOCK;
+constexpr int MAX_PAYOFFS = 16;
+constexpr int MAX_LANES = 64;
+
+// Per-lane scheme constants, precomputed on the host in FP64 (the reference's
+// SDEScheme constructors, MSchemes.hpp:104-122, :199-216, :296-352, :485-491)
+// and stored in the kernel's precision.
+template <typename Real>
+struct LaneC {
+    Real S0, v0;
+    Real rqdt;                       // (r - q') dt, q' includes the Bates compensator
+    Real half_dt, sqrt_dt, rho, rho_c;
+    Real omkdt, ktdt, ssdt, qs2dt, iden;     // Euler / implicit Milstein
+    Real ekdt, tmf, s2f1, s2f2, K2, K3, K4, A, cv, eps;   // Andersen QE (cv = -K3/2)
+    Real eml_dt, mu_j, sigma_j;      // Bates: exp(-lambda dt), jump mean and vol
+};
+
+template <typename Real>
+struct PayD {
+    int32_t type, right;
+    Real K, B;
+};
+
+template <typename Real>
+struct PathArgs {
+    const LaneC<Real> *lanes;   // [n_lanes padded to a multiple of L]
+    const PayD<Real> *pays;     // [n_pay]
+    int32_t n_lanes, n_pay, n_steps, pad0;
+    int64_t n_paths, n_tiles;
+    uint32_t key0, key1;
+    Real itm_eps;
+    double *part_sum;           // [(lane*n_pay + j) * n_tiles + tile]
+    double *part_m2;
+    uint32_t *part_itm;
+    Real *store;                // [step][path] or null
+};
+
+}  // namespace qk

@@ -1,0 +1,119 @@
+"""Python handle on the C ABI, used by the tests and bench.py.
+
+Every call goes to libquantkos.so on the GPU; nothing here computes prices.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Bumps, Contract, Greeks, Payoff, Stats, Timing
+
+
+class QKError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__("%s (%d): %s" % (_lib.load().qk_strerror(status).decode(), status, message))
+        self.status = status
+
+
+def contract(**kw) -> Contract:
+    """A qk_contract.  Defaults: the reference's Config.hpp defaults where it has
+    them (r=0.05, q=0, kappa=2, theta=0.04, sigma=0.3, rho=-0.7, Bates jumps
+    0.1/-0.1/0.15, seed 184467440737095), S0 = 1 (normalised), FP32, Euler."""
+    c = Contract()
+    d = dict(model=_lib.HESTON, scheme=_lib.EULER, precision=_lib.FP32, n_steps=252, n_paths=100_000,
+             seed=184467440737095, S0=1.0, v0=0.04, r=0.05, q=0.0, T=1.0, kappa=2.0, theta=0.04,
+             sigma=0.3, rho=-0.7, lambda_j=0.1, mu_j=-0.1, sigma_j=0.15, price_scale=1.0)
+    d.update(kw)
+    for k, v in d.items():
+        setattr(c, k, v)
+    return c
+
+
+def payoff(type_, right=_lib.CALL, K=1.0, B=1.0) -> Payoff:
+    return Payoff(type_, right, K, B)
+
+
+def bumps(**kw) -> Bumps:
+    """qk_bumps with the reference's default bump sizes (Config.hpp:443-446)."""
+    b = Bumps()
+    d = dict(delta_gamma=1, vega_vomma=1, vanna=1, rho=1, theta=1, theta_central=0,
+             rel_S=0.001, abs_vol=0.01, abs_r=0.0001, abs_T=1.0 / 365.0)
+    d.update(kw)
+    for k, v in d.items():
+        setattr(b, k, v)
+    return b
+
+
+class Engine:
+    """One qk_ctx: a device, a memory budget and a stream."""
+
+    def __init__(self, device: int = 0, mem_budget_bytes: int = 0):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        st = self.L.qk_ctx_create(device, mem_budget_bytes, C.byref(h))
+        if st != 0:
+            raise QKError(st, "qk_ctx_create(device=%d)" % device)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.qk_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st):
+        if st != 0:
+            raise QKError(st, self.L.qk_last_error(self.h).decode())
+
+    def price(self, c: Contract, p: Payoff) -> Stats:
+        out = Stats()
+        self._check(self.L.qk_price(self.h, C.byref(c), C.byref(p), C.byref(out)))
+        return out
+
+    def price_payoffs(self, c: Contract, payoffs) -> list:
+        arr = (Payoff * len(payoffs))(*payoffs)
+        out = (Stats * len(payoffs))()
+        self._check(self.L.qk_price_payoffs(self.h, C.byref(c), arr, len(payoffs), out))
+        return list(out)
+
+    def price_lanes(self, lanes, payoffs) -> list:
+        la = (Contract * len(lanes))(*lanes)
+        pa = (Payoff * len(payoffs))(*payoffs)
+        out = (Stats * (len(lanes) * len(payoffs)))()
+        self._check(self.L.qk_price_lanes(self.h, la, len(lanes), pa, len(payoffs), out))
+        n = len(payoffs)
+        return [list(out[i * n:(i + 1) * n]) for i in range(len(lanes))]
+
+    def greeks(self, c: Contract, payoffs, b: Bumps) -> list:
+        pa = (Payoff * len(payoffs))(*payoffs)
+        out = (Greeks * len(payoffs))()
+        self._check(self.L.qk_price_greeks(self.h, C.byref(c), pa, len(payoffs), C.byref(b), out))
+        return list(out)
+
+    def price_american(self, c: Contract, right: int, K: float, want_coeffs: bool = False):
+        out = Stats()
+        co = np.zeros((c.n_steps, 4)) if want_coeffs else None
+        ptr = co.ctypes.data_as(C.POINTER(C.c_double)) if want_coeffs else None
+        self._check(self.L.qk_price_american(self.h, C.byref(c), right, K, C.byref(out), ptr))
+        return (out, co) if want_coeffs else out
+
+    def simulate_paths(self, c: Contract) -> np.ndarray:
+        S = np.empty((c.n_steps, c.n_paths), dtype=np.float64)
+        self._check(self.L.qk_simulate_paths(self.h, C.byref(c), S.ctypes.data_as(C.POINTER(C.c_double))))
+        return S
+
+    def timing(self) -> Timing:
+        t = Timing()
+        self._check(self.L.qk_last_timing(self.h, C.byref(t)))
+        return t
+
+    def synchronize(self):
+        self._check(self.L.qk_synchronize(self.h))

@@ -1,0 +1,291 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on the
+same counter-based random numbers, plus the reference's closed-form checks.
+
+Tolerances (written here, per test):
+  * FP64 per-path spot values: max relative difference 1e-9 (libm vs device
+    math ulps over <= 100 steps);
+  * FP32 per-path: median relative difference 5e-6, < 1% of paths beyond 1e-4
+    (rare QE branch flips near psi_c or U = p are allowed);
+  * FP64 prices / moments: relative 1e-9; FP32 prices: 0.05 standard errors;
+  * closed forms: 3 standard errors, the reference's rule (TestsUtils.hpp:92-115).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import quantkos_amd as qk
+from quantkos_amd import engine as qe
+
+pytestmark = pytest.mark.gpu
+
+MODELS = [(qk.HESTON, qk.EULER), (qk.HESTON, qk.MILSTEIN), (qk.HESTON, qk.QE),
+          (qk.BATES, qk.EULER), (qk.BATES, qk.MILSTEIN), (qk.BATES, qk.QE)]
+
+
+def omodel(o, c):
+    return o.model(model=c.model, scheme=c.scheme, fp32_semantics=int(c.precision == qk.FP32),
+                   n_steps=c.n_steps, seed=c.seed, S0=c.S0, v0=c.v0, r=c.r, q=c.q, T=c.T,
+                   kappa=c.kappa, theta=c.theta, sigma=c.sigma, rho=c.rho, lambda_j=c.lambda_j,
+                   mu_j=c.mu_j, sigma_j=c.sigma_j)
+
+
+def base_contract(model, scheme, precision, **kw):
+    d = dict(model=model, scheme=scheme, precision=precision, n_steps=60, n_paths=1000, seed=42,
+             S0=1.0, v0=0.04, r=0.03, q=0.01, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7,
+             lambda_j=0.5, mu_j=-0.05, sigma_j=0.1)
+    d.update(kw)
+    return qe.contract(**d)
+
+
+def opay(o, p):
+    return o.Payoff(p.type, p.right, p.K, p.B)
+
+
+@pytest.mark.parametrize("model,scheme", MODELS)
+def test_paths_fp64_match_oracle(engine, oracle, model, scheme):
+    c = base_contract(model, scheme, qk.FP64)
+    g = engine.simulate_paths(c)
+    o = oracle.simulate(omodel(oracle, c), c.n_paths, store=True)["paths"]
+    rel = np.abs(g - o) / np.abs(o)
+    assert np.isfinite(g).all()
+    assert rel.max() < 1e-9, (rel.max(), np.unravel_index(rel.argmax(), rel.shape))
+
+
+@pytest.mark.parametrize("model,scheme", MODELS)
+def test_paths_fp32_match_oracle(engine, oracle, model, scheme):
+    c = base_contract(model, scheme, qk.FP32)
+    g = engine.simulate_paths(c)
+    o = oracle.simulate(omodel(oracle, c), c.n_paths, store=True)["paths"]
+    rel = np.abs(g - o) / np.abs(o)
+    assert np.isfinite(g).all()
+    final = rel[-1]
+    assert np.median(final) < 5e-6, np.median(final)
+    assert (final > 1e-4).mean() < 0.01, (final > 1e-4).mean()
+
+
+def all_payoffs(right):
+    P = qe.payoff
+    up, dn = 1.2, 0.85
+    return [P(qk.EUROPEAN, right, 1.0), P(qk.ASIAN, right, 1.0), P(qk.BARRIER_UP_OUT, right, 1.0, up),
+            P(qk.BARRIER_DOWN_OUT, right, 1.0, dn), P(qk.BARRIER_UP_IN, right, 1.0, up),
+            P(qk.BARRIER_DOWN_IN, right, 1.0, dn), P(qk.LOOKBACK_FLOATING, right, 1.0),
+            P(qk.LOOKBACK_FIXED, right, 1.05), P(qk.BINARY_CASH, right, 1.05),
+            P(qk.BINARY_ASSET, right, 0.95)]
+
+
+@pytest.mark.parametrize("model,scheme", [(qk.HESTON, qk.QE), (qk.BATES, qk.EULER)])
+@pytest.mark.parametrize("right", [qk.CALL, qk.PUT])
+def test_prices_fp64_match_oracle(engine, oracle, model, scheme, right):
+    c = base_contract(model, scheme, qk.FP64, n_paths=20_011, n_steps=50, price_scale=100.0)
+    pays = all_payoffs(right)
+    g = engine.price_payoffs(c, pays)
+    o = oracle.price(omodel(oracle, c), c.n_paths, [opay(oracle, p) for p in pays], 100.0)
+    for gp, op, p in zip(g, o, pays):
+        assert gp.n == op.n == c.n_paths
+        assert gp.itm == op.itm, (p.type, gp.itm, op.itm)
+        assert math.isclose(gp.price, op.price, rel_tol=1e-9, abs_tol=1e-12), (p.type, gp.price, op.price)
+        assert math.isclose(gp.se, op.se, rel_tol=1e-8, abs_tol=1e-12), (p.type, gp.se, op.se)
+
+
+def c1_payoffs():
+    P = qe.payoff
+    return [P(qk.ASIAN, qk.CALL, 1.0), P(qk.BARRIER_UP_OUT, qk.CALL, 1.0, 1.3),
+            P(qk.LOOKBACK_FLOATING, qk.PUT, 1.0)]
+
+
+@pytest.mark.parametrize("model,scheme", MODELS)
+def test_prices_fp32_within_se_of_oracle(engine, oracle, model, scheme):
+    c = base_contract(model, scheme, qk.FP32, n_paths=50_000, n_steps=100)
+    pays = c1_payoffs() + [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0)]
+    g = engine.price_payoffs(c, pays)
+    o = oracle.price(omodel(oracle, c), c.n_paths, [opay(oracle, p) for p in pays])
+    for gp, op in zip(g, o):
+        assert abs(gp.price - op.price) < 0.05 * op.se, (gp.price, op.price, op.se)
+
+
+def test_fp32_matches_fp64_gpu_on_same_numbers(engine):
+    """Config-1 shape at reduced size: FP32 and FP64 GPU runs share the random
+    stream (24-bit vs 32-bit uniforms of the same Philox words)."""
+    c32 = base_contract(qk.HESTON, qk.QE, qk.FP32, n_paths=200_000, n_steps=365, S0=1.0, v0=0.04,
+                        r=0.03, q=0.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7)
+    c64 = base_contract(qk.HESTON, qk.QE, qk.FP64, n_paths=200_000, n_steps=365, S0=1.0, v0=0.04,
+                        r=0.03, q=0.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7)
+    g32 = engine.price_payoffs(c32, c1_payoffs())
+    g64 = engine.price_payoffs(c64, c1_payoffs())
+    for a, b in zip(g32, g64):
+        assert abs(a.price - b.price) < 1.0 * b.se
+
+
+def test_deterministic_and_ragged(engine):
+    c = base_contract(qk.BATES, qk.QE, qk.FP32, n_paths=12_345, n_steps=40)
+    a = engine.price_payoffs(c, c1_payoffs())
+    b = engine.price_payoffs(c, c1_payoffs())
+    for x, y in zip(a, b):
+        assert (x.price, x.se, x.itm) == (y.price, y.se, y.itm)
+    # a prefix of the path set prices the first paths identically: n=1 path
+    c.n_paths = 1
+    one = engine.price(c, qe.payoff(qk.EUROPEAN, qk.CALL, 0.5))
+    assert one.n == 1 and one.se == 0.0
+
+
+@pytest.mark.parametrize("precision", [qk.FP32, qk.FP64])
+def test_lanes_match_single_launches(engine, precision):
+    base = base_contract(qk.HESTON, qk.MILSTEIN, precision, n_paths=30_000, n_steps=50)
+    lanes = []
+    for i, (S0, v0, r, T) in enumerate([(1.0, 0.04, 0.03, 1.0), (1.01, 0.04, 0.03, 1.0),
+                                        (1.0, 0.05, 0.03, 1.0), (1.0, 0.04, 0.031, 1.0),
+                                        (1.0, 0.04, 0.03, 0.99), (0.99, 0.03, 0.02, 0.9)]):
+        lanes.append(base_contract(qk.HESTON, qk.MILSTEIN, precision, n_paths=30_000, n_steps=50,
+                                   S0=S0, v0=v0, r=r, T=T))
+    pays = [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0), qe.payoff(qk.ASIAN, qk.PUT, 1.0)]
+    multi = engine.price_lanes(lanes, pays)
+    tol = 1e-12 if precision == qk.FP64 else 2e-6
+    for lane, row in zip(lanes, multi):
+        single = engine.price_payoffs(lane, pays)
+        for a, b in zip(row, single):
+            assert math.isclose(a.price, b.price, rel_tol=tol, abs_tol=1e-12), (a.price, b.price)
+
+
+def test_greeks_fp64_match_oracle(engine, oracle):
+    c = base_contract(qk.HESTON, qk.MILSTEIN, qk.FP64, n_paths=20_000, n_steps=50, price_scale=100.0)
+    b = qe.bumps(rel_S=0.01, abs_vol=0.02, abs_r=0.001, abs_T=0.02)
+    pays = [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0), qe.payoff(qk.BINARY_CASH, qk.CALL, 1.05)]
+    g = engine.greeks(c, pays, b)
+    ob = oracle.Bumps(1, 1, 1, 1, 1, 0, 0.01, 0.02, 0.001, 0.02)
+    for gp, p in zip(g, pays):
+        og = oracle.greeks(omodel(oracle, c), c.n_paths, opay(oracle, p), ob, 100.0)
+        assert gp.n_lanes == og.n_pricings == 12
+        for k in ("delta", "gamma", "vega", "vomma", "vanna", "rho", "theta", "base_price"):
+            assert math.isclose(getattr(gp, k), getattr(og, k), rel_tol=1e-6, abs_tol=1e-9), \
+                (k, getattr(gp, k), getattr(og, k))
+
+
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+@pytest.mark.parametrize("model,scheme", [(qk.HESTON, qk.MILSTEIN), (qk.BATES, qk.QE)])
+def test_american_matches_oracle(engine, oracle, precision, model, scheme):
+    """Same paths, same regression.  The Laguerre Gram at early dates is badly
+    conditioned (X = S/K clusters), so summation-order rounding moves individual
+    coefficients; the fitted continuation values and the price must agree:
+    FP64 within 0.02 SE, FP32 (24-bit uniforms, FP32 paths) within 0.5 SE."""
+    c = base_contract(model, scheme, precision, n_paths=20_000, n_steps=50, price_scale=100.0)
+    g, gco = engine.price_american(c, qk.PUT, 1.1, want_coeffs=True)
+    o, oco = oracle.price_american(omodel(oracle, c), c.n_paths, qk.PUT, 1.1, 100.0, want_coeffs=True)
+    assert g.n == o.n == c.n_paths
+    if precision == qk.FP64:
+        # continuation value sum_k C_k L_k(X) over the in-the-money range of X = S/K
+        X = np.linspace(0.7, 1.0, 31)
+        Lx = np.stack([oracle.laguerre4(x) for x in X])
+        cont_g, cont_o = Lx @ gco[:-1].T, Lx @ oco[:-1].T
+        assert np.abs(cont_g - cont_o).max() < 2e-3, np.abs(cont_g - cont_o).max()
+        assert np.allclose(gco[-10:-1], oco[-10:-1], rtol=1e-6, atol=1e-9)
+        assert abs(g.price - o.price) < 0.02 * o.se, (g.price, o.price, o.se)
+    else:
+        assert abs(g.price - o.price) < 0.5 * o.se, (g.price, o.price, o.se)
+    # American put >= European put and >= intrinsic
+    eu = engine.price(c, qe.payoff(qk.EUROPEAN, qk.PUT, 1.1))
+    assert g.price >= eu.price - 3 * eu.se
+    assert g.price >= 0.1 * 100.0 - 1e-9
+
+
+IKONEN_TOIVANEN = {8.0: 2.0, 9.0: 1.107641, 10.0: 0.520030, 11.0: 0.213668, 12.0: 0.082036}
+
+
+@pytest.mark.parametrize("precision", [qk.FP32, qk.FP64])
+@pytest.mark.parametrize("S0", sorted(IKONEN_TOIVANEN))
+def test_american_ikonen_toivanen(engine, precision, S0):
+    """AmericanOption.hpp:109-177 at the reference's own size: Heston implicit
+    Milstein, dt = T/1000, 500,000 paths, tolerance 3 SE + max(2% ref, 0.005)."""
+    c = qe.contract(model=qk.HESTON, scheme=qk.MILSTEIN, precision=precision, n_steps=1000,
+                    n_paths=500_000, seed=184467440737095, S0=1.0, v0=0.0625, r=0.1, q=0.0, T=0.25,
+                    kappa=5.0, theta=0.16, sigma=0.9, rho=0.1, price_scale=S0)
+    s = engine.price_american(c, qk.PUT, 10.0 / S0)
+    ref = IKONEN_TOIVANEN[S0]
+    assert abs(s.price - ref) <= 3 * s.se + max(0.02 * ref, 0.005), (s.price, ref, s.se)
+
+
+@pytest.mark.parametrize("model,scheme", MODELS)
+@pytest.mark.parametrize("precision", [qk.FP32, qk.FP64])
+def test_no_noise_drift(engine, model, scheme, precision):
+    """NoNoiseTest.hpp:113-204: zero volatility gives S_T = S0 exp((r-q)T) on every path."""
+    c = qe.contract(model=model, scheme=scheme, precision=precision, n_steps=365, n_paths=100,
+                    S0=1.0, v0=0.0, r=0.05, q=0.02, T=1.0, kappa=0.0, theta=0.0, sigma=0.0, rho=0.0,
+                    lambda_j=0.0, mu_j=0.0, sigma_j=0.0, price_scale=100.0)
+    S = engine.simulate_paths(c) * 100.0
+    tol = 1e-8 if precision == qk.FP64 else 5e-3
+    assert np.abs(S[-1] - 100.0 * math.exp(0.03)).max() < tol
+
+
+@pytest.mark.parametrize("model,scheme", MODELS)
+def test_black_scholes_limit(engine, oracle, model, scheme):
+    """BlackScholesTest.hpp:46-107: kappa=theta=sigma=rho=0 (lambda=0) gives Black-Scholes."""
+    c = qe.contract(model=model, scheme=scheme, precision=qk.FP64, n_steps=64, n_paths=1_000_000,
+                    seed=7, S0=1.0, v0=0.04, r=0.04, q=0.0, T=1.0, kappa=0.0, theta=0.0, sigma=0.0,
+                    rho=0.0, lambda_j=0.0, mu_j=0.0, sigma_j=0.0, price_scale=100.0)
+    s = engine.price(c, qe.payoff(qk.EUROPEAN, qk.CALL, 1.0))
+    exact = oracle.bs_call(100.0, 100.0, 1.0, 0.04, 0.0, 0.2)
+    assert abs(s.price - exact) <= 3.0 * s.se, (s.price, exact, s.se)
+
+
+@pytest.mark.parametrize("scheme", [qk.EULER, qk.MILSTEIN, qk.QE])
+def test_heston_closed_form(engine, oracle, scheme):
+    """HestonTest.hpp:49-117 (Rouah p.28, 6.2528) at the finest grid of the reference (N=512)."""
+    c = qe.contract(model=qk.HESTON, scheme=scheme, precision=qk.FP32, n_steps=512, n_paths=2_000_000,
+                    seed=11, S0=1.0, v0=0.05, r=0.03, q=0.02, T=0.5, kappa=5.0, theta=0.05, sigma=0.5,
+                    rho=-0.8, price_scale=100.0)
+    s = engine.price(c, qe.payoff(qk.EUROPEAN, qk.CALL, 1.0))
+    exact = oracle.heston_call(100.0, 100.0, 0.5, 0.03, 0.02, 0.05, 5.0, 0.05, 0.5, -0.8)
+    assert abs(exact - 6.2528) < 2e-4
+    assert abs(s.price - exact) <= 3.0 * s.se, (s.price, exact, s.se)
+
+
+@pytest.mark.parametrize("scheme", [qk.EULER, qk.MILSTEIN, qk.QE])
+def test_bates_closed_form(engine, oracle, scheme):
+    """BatesTest.hpp:52-123 (lambda=0.11, muJ=-0.15, sigmaJ=0.11) at N=512."""
+    c = qe.contract(model=qk.BATES, scheme=scheme, precision=qk.FP32, n_steps=512, n_paths=2_000_000,
+                    seed=13, S0=1.0, v0=0.05, r=0.03, q=0.02, T=0.5, kappa=5.0, theta=0.05, sigma=0.5,
+                    rho=-0.8, lambda_j=0.11, mu_j=-0.15, sigma_j=0.11, price_scale=100.0)
+    s = engine.price(c, qe.payoff(qk.EUROPEAN, qk.CALL, 1.0))
+    exact = oracle.bates_call(100.0, 100.0, 0.5, 0.03, 0.02, 0.05, 5.0, 0.05, 0.5, -0.8, 0.11, -0.15, 0.11)
+    assert abs(s.price - exact) <= 3.0 * s.se, (s.price, exact, s.se)
+
+
+def test_readme_bench_golden(engine):
+    """bench_asian_option_price_no_greeks.yaml:65-67: 5.110195 +- 0.003025 (QE Asian, 5M x 365)."""
+    c = qe.contract(model=qk.HESTON, scheme=qk.QE, precision=qk.FP32, n_steps=365, n_paths=5_000_000,
+                    seed=88471920573105, S0=1.0, v0=0.05, r=0.03, q=0.02, T=1.0, kappa=5.0,
+                    theta=0.05, sigma=0.5, rho=-0.8, price_scale=100.0)
+    s = engine.price(c, qe.payoff(qk.ASIAN, qk.CALL, 1.0))
+    combined = math.sqrt(s.se ** 2 + 0.003025 ** 2)
+    assert abs(s.price - 5.110195) <= 3.0 * combined, (s.price, s.se)
+    assert abs(s.prob_itm - 0.5409264) < 0.005
+
+
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+def test_greek_put_call_parity(engine, precision):
+    """PutCallParityTest.hpp:194-277 at its own size and bumps (1M paths, dt=T/365,
+    Heston implicit Milstein), with compute_greek_tol (:120-132)."""
+    c = qe.contract(model=qk.HESTON, scheme=qk.MILSTEIN, precision=precision, n_steps=365,
+                    n_paths=1_000_000, seed=184467440737095, S0=1.0, v0=0.04, r=0.05, q=0.0, T=1.0,
+                    kappa=2.0, theta=0.04, sigma=0.3, rho=-0.7, price_scale=100.0)
+    if precision == qk.FP32:
+        b = qe.bumps(rel_S=0.02, abs_vol=0.03, abs_r=0.005, abs_T=7.0 / 365.0)
+    else:
+        b = qe.bumps(rel_S=0.001, abs_vol=0.005, abs_r=0.0001, abs_T=1.0 / 365.0)
+    gc, gp = engine.greeks(c, [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0), qe.payoff(qk.EUROPEAN, qk.PUT, 1.0)], b)
+    K, T, r = 100.0, 1.0, 0.05
+    se = gc.base_se
+
+    def tol(scale, mult):
+        return max(0.01 * max(abs(scale), 1.0), mult * se, 1e-4)
+
+    assert abs((gc.delta - gp.delta) - 1.0) <= tol(1.0, 3.3)
+    assert abs(gc.gamma - gp.gamma) <= tol(1.0, 3.3)
+    assert abs(gc.vega - gp.vega) <= tol(1.0, 3.3)
+    assert abs(gc.vomma - gp.vomma) <= tol(1.0, 8.0)
+    assert abs(gc.vanna - gp.vanna) <= tol(1.0, 3.3)
+    exp_rho = K * T * math.exp(-r * T)
+    assert abs((gc.rho - gp.rho) - exp_rho) <= tol(exp_rho, 3.3)
+    exp_theta = -r * K * math.exp(-r * T)
+    assert abs((gc.theta - gp.theta) - exp_theta) <= tol(exp_theta, 5.0)
+    assert gc.n_lanes == 12

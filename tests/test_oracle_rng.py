@@ -1,0 +1,51 @@
+"""Pins the oracle's random stream: Philox4x32-10 known-answer vectors, the
+uniform maps, and the reference's normal-moment test."""
+import numpy as np
+
+
+# Known-answer vectors of Philox4x32-10 published with Random123 (kat_vectors):
+# (counter, key) -> output.
+KAT = [
+    ((0x00000000, 0x00000000, 0x00000000, 0x00000000), (0x00000000, 0x00000000),
+     (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff), (0xffffffff, 0xffffffff),
+     (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+def test_philox_known_answers(oracle):
+    for ctr, key, want in KAT:
+        assert tuple(oracle.philox(ctr, key)) == want
+
+
+def test_uniform_maps_open_interval(oracle):
+    assert oracle.lib().qko_uniform(0, 0) == 2.0 ** -33
+    assert oracle.lib().qko_uniform(0xFFFFFFFF, 0) == 1.0 - 2.0 ** -33
+    assert oracle.lib().qko_uniform(0, 1) == 2.0 ** -25
+    assert oracle.lib().qko_uniform(0xFFFFFFFF, 1) == 1.0 - 2.0 ** -25
+    # u24 keeps the top 24 bits: words differing only in the low byte map together
+    assert oracle.lib().qko_uniform(0x12345600, 1) == oracle.lib().qko_uniform(0x123456FF, 1)
+
+
+def test_normal_moments(oracle):
+    """GaussianRNGenTest.hpp:200-208: |mean| < 0.005, |var - 1| < 0.01."""
+    for u24 in (False, True):
+        z = oracle.normals(184467440737095, 20_000, 50, u24=u24)
+        for k in (0, 1):
+            x = z[:, :, k].ravel()
+            assert abs(x.mean()) < 0.005
+            assert abs(x.var() - 1.0) < 0.01
+        # the two Box-Muller outputs are uncorrelated, as are consecutive steps
+        assert abs(np.corrcoef(z[:, :, 0].ravel(), z[:, :, 1].ravel())[0, 1]) < 0.005
+        assert abs(np.corrcoef(z[:, :-1, 0].ravel(), z[:, 1:, 0].ravel())[0, 1]) < 0.005
+        # fourth moment of a standard normal is 3
+        assert abs((z[:, :, 0] ** 4).mean() - 3.0) < 0.05
+
+
+def test_streams_depend_on_seed_path_and_step(oracle):
+    a = oracle.normals(1, 4, 3)
+    b = oracle.normals(2, 4, 3)
+    assert not np.allclose(a, b)
+    assert len(np.unique(a[:, :, 0])) == 12
