@@ -137,10 +137,38 @@ qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk_payoff *pa
 qk_status qk_price_american(qk_ctx *ctx, const qk_contract *c, int32_t right, double K,
                             qk_stats *out, double *coeffs_out);
 
-/* Simulated spot paths, copied to host memory laid out [step][path]
- * (S after step t+1 at row t), as the reference's StorePaths kernel writes
- * (MSolver.hpp:117-119). */
-qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, double *S_out);
+/* Simulated spot paths [path0, path0 + c->n_paths) of the path set, copied to
+ * host memory laid out [step][path] (S after step t+1 at row t), as the
+ * reference's StorePaths kernel writes them (MSolver.hpp:117-119).  Path p
+ * draws the same random numbers whatever path0 and n_paths are. */
+qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, int64_t path0, double *S_out);
+
+/* ---- host layer: QuantKos YAML configuration files ---- */
+typedef struct {
+    qk_contract contract;     /* after the Normalize_prices scaling                 */
+    qk_payoff payoff;         /* type QK_AMERICAN for American options              */
+    qk_bumps bumps;           /* the Greeks requested (all switches 0 if none)      */
+    int32_t compute_greeks;
+    int32_t write_paths;      /* Output.Name_Paths_Out_File is not empty            */
+} qk_run_spec;
+
+/* Config::Parser::parse (IO/InputParser.hpp:56, src/IO/ConfigParser.cpp:36-294)
+ * followed by UInputs::apply_price_scaling (Config.hpp:710-723): reads,
+ * validates and scales a configuration file.  No device work.  On failure the
+ * reference's error text is copied to err (if not NULL). */
+qk_status qk_config_load(const char *yaml_path, qk_run_spec *out, char *err, size_t err_len);
+
+typedef struct {
+    qk_stats stats;            /* price, se, prob_itm, n, itm                       */
+    qk_greeks greeks;          /* zero unless Greeks were requested                 */
+    double t_total_s, t_pricing_s, t_greeks_s;   /* the reference's MCTimer lines  */
+} qk_run_result;
+
+/* MCDispatcher(Parser::parse(file)).launch_montecarlo() (app/main.cpp:78-93,
+ * Distpatcher.hpp:78-83) on the given device; prints the reference's report
+ * unless quiet. */
+qk_status qk_run_config(const char *yaml_path, int device, int32_t quiet, qk_run_result *out, char *err,
+                        size_t err_len);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,16 @@ class Timing(C.Structure):
                 ("path_steps", C.c_int64)]
 
 
+class RunSpec(C.Structure):
+    _fields_ = [("contract", Contract), ("payoff", Payoff), ("bumps", Bumps),
+                ("compute_greeks", C.c_int32), ("write_paths", C.c_int32)]
+
+
+class RunResult(C.Structure):
+    _fields_ = [("stats", Stats), ("greeks", Greeks), ("t_total_s", C.c_double),
+                ("t_pricing_s", C.c_double), ("t_greeks_s", C.c_double)]
+
+
 _lib = None
 
 
@@ -86,7 +96,9 @@ def load():
         "qk_price_lanes": ([vp, P(Contract), C.c_int32, P(Payoff), C.c_int32, P(Stats)], st),
         "qk_price_greeks": ([vp, P(Contract), P(Payoff), C.c_int32, P(Bumps), P(Greeks)], st),
         "qk_price_american": ([vp, P(Contract), C.c_int32, C.c_double, P(Stats), P(C.c_double)], st),
-        "qk_simulate_paths": ([vp, P(Contract), P(C.c_double)], st),
+        "qk_simulate_paths": ([vp, P(Contract), C.c_int64, P(C.c_double)], st),
+        "qk_config_load": ([C.c_char_p, P(RunSpec), C.c_char_p, C.c_size_t], st),
+        "qk_run_config": ([C.c_char_p, C.c_int, C.c_int32, P(RunResult), C.c_char_p, C.c_size_t], st),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -190,6 +190,7 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
     size_t b = groups * n_tiles * (8 + 8 + 4);          // tile moments
     b += (size_t)(n_lanes + 4) * sizeof(LaneC<double>) + MAX_PAYOFFS * sizeof(PayD<double>);
     b += groups * (sizeof(qk_stats) + 16) + 4096;
+    b += groups * (size_t)merge_chunks((int64_t)n_tiles) * MERGE_MOM_BYTES;
     if (american) {
         b += (size_t)c.n_paths * (size_t)c.n_steps * rs;   // path store [date][path]
         b += (size_t)c.n_paths * rs;                         // cash flows
@@ -251,7 +252,7 @@ extern "C" qk_status qk_ctx_create(int device, size_t mem_budget_bytes, qk_ctx *
 qk_ctx::~qk_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (DevBuf *b : {&lanes, &pays, &psum, &pm2, &pitm, &disc, &scale, &stats, &greeks, &store, &cf, &coeffs,
-                      &bpart, &counter, &sums})
+                      &bpart, &counter, &sums, &mscratch})
         b->release();
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
@@ -316,6 +317,7 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     if ((st = ctx->pm2.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pitm.reserve(ctx, groups * n_tiles * 4)) != QK_OK) return st;
     if ((st = ctx->stats.reserve(ctx, groups * sizeof(qk_stats))) != QK_OK) return st;
+    if ((st = ctx->mscratch.reserve(ctx, groups * merge_chunks(n_tiles) * MERGE_MOM_BYTES)) != QK_OK) return st;
     hipStream_t s = ctx->stream;
     QK_HIP(hipMemcpyAsync(ctx->lanes.p, hl.data(), hl.size() * sizeof(LaneC<Real>), hipMemcpyHostToDevice, s));
     QK_HIP(hipMemcpyAsync(ctx->pays.p, hp.data(), hp.size() * sizeof(PayD<Real>), hipMemcpyHostToDevice, s));
@@ -330,6 +332,7 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     a.n_steps = c0.n_steps;
     a.n_paths = c0.n_paths;
     a.n_tiles = n_tiles;
+    a.path0 = 0;
     a.key0 = (uint32_t)c0.seed;
     a.key1 = (uint32_t)(c0.seed >> 32);
     a.itm_eps = (Real)(c0.precision == QK_FP32 ? 1e-6 : 1e-12);   // OptionPricer.hpp:316
@@ -342,7 +345,8 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     QK_HIP(launch_paths<Real>(a, c0.model, c0.scheme, pathdep, false, L, s));
     QK_HIP(hipEventRecord(ctx->ev[1], s));
     QK_HIP(launch_merge(a.part_sum, a.part_m2, a.part_itm, n_tiles, c0.n_paths, n_lanes, n_pay,
-                        (const double *)ctx->disc.p, (const double *)ctx->scale.p, (qk_stats *)ctx->stats.p, s));
+                        (const double *)ctx->disc.p, (const double *)ctx->scale.p, ctx->mscratch.p,
+                        (qk_stats *)ctx->stats.p, s));
     QK_HIP(hipEventRecord(ctx->ev[2], s));
     return QK_OK;
 }
@@ -485,7 +489,7 @@ extern "C" qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk
 namespace qk {
 
 template <typename Real>
-static qk_status generate_store(qk_ctx *ctx, const qk_contract &c) {
+static qk_status generate_store(qk_ctx *ctx, const qk_contract &c, int64_t path0 = 0) {
     const LaneC<Real> hl = make_lane<Real>(c);
     qk_status st;
     if ((st = ctx->lanes.reserve(ctx, sizeof(LaneC<Real>))) != QK_OK) return st;
@@ -499,6 +503,7 @@ static qk_status generate_store(qk_ctx *ctx, const qk_contract &c) {
     a.n_steps = c.n_steps;
     a.n_paths = c.n_paths;
     a.n_tiles = (c.n_paths + TILE - 1) / TILE;
+    a.path0 = path0;
     a.key0 = (uint32_t)c.seed;
     a.key1 = (uint32_t)(c.seed >> 32);
     a.store = (Real *)ctx->store.p;
@@ -526,6 +531,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     if ((st = ctx->disc.reserve(ctx, 8)) != QK_OK) return st;
     if ((st = ctx->scale.reserve(ctx, 8)) != QK_OK) return st;
     if ((st = ctx->stats.reserve(ctx, sizeof(qk_stats))) != QK_OK) return st;
+    if ((st = ctx->mscratch.reserve(ctx, merge_chunks(n_tiles) * MERGE_MOM_BYTES)) != QK_OK) return st;
     hipStream_t s = ctx->stream;
     const double one = 1.0, scale = c.price_scale;
     QK_HIP(hipMemcpyAsync(ctx->disc.p, &one, 8, hipMemcpyHostToDevice, s));
@@ -555,7 +561,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     QK_HIP(hipEventRecord(ctx->ev[3], s));
     QK_HIP(launch_merge((const double *)ctx->psum.p, (const double *)ctx->pm2.p, (const uint32_t *)ctx->pitm.p,
                         n_tiles, n, 1, 1, (const double *)ctx->disc.p, (const double *)ctx->scale.p,
-                        (qk_stats *)ctx->stats.p, s));
+                        ctx->mscratch.p, (qk_stats *)ctx->stats.p, s));
     QK_HIP(hipEventRecord(ctx->ev[2], s));
     QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, s));
     if (coeffs_out) QK_HIP(hipMemcpyAsync(coeffs_out, ctx->coeffs.p, (size_t)N * 4 * 8, hipMemcpyDeviceToHost, s));
@@ -593,19 +599,20 @@ extern "C" qk_status qk_price_american(qk_ctx *ctx, const qk_contract *c, int32_
     return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, true);
 }
 
-extern "C" qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, double *S_out) {
+extern "C" qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, int64_t path0, double *S_out) {
     if (!ctx || !S_out) return fail(ctx, QK_EINVAL, "null argument");
+    if (path0 < 0) return fail(ctx, QK_EINVAL, "path0 must be >= 0");
     qk_status st = validate_contract(ctx, c);
     if (st != QK_OK) return st;
     if ((st = check_budget(ctx, footprint(*c, 1, 1, true))) != QK_OK) return st;
     QK_HIP(hipSetDevice(ctx->device));
     const size_t n = (size_t)c->n_paths * c->n_steps;
     if (c->precision == QK_FP64) {
-        if ((st = generate_store<double>(ctx, *c)) != QK_OK) return st;
+        if ((st = generate_store<double>(ctx, *c, path0)) != QK_OK) return st;
         QK_HIP(hipMemcpyAsync(S_out, ctx->store.p, n * 8, hipMemcpyDeviceToHost, ctx->stream));
         QK_HIP(hipStreamSynchronize(ctx->stream));
     } else {
-        if ((st = generate_store<float>(ctx, *c)) != QK_OK) return st;
+        if ((st = generate_store<float>(ctx, *c, path0)) != QK_OK) return st;
         std::vector<float> tmp(n);
         QK_HIP(hipMemcpyAsync(tmp.data(), ctx->store.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
         QK_HIP(hipStreamSynchronize(ctx->stream));

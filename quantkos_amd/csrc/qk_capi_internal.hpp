@@ -37,6 +37,6 @@ struct qk_ctx {
     hipEvent_t ev[4] = {};
     std::string err;
     qk_timing timing{};
-    qk::DevBuf lanes, pays, psum, pm2, pitm, disc, scale, stats, greeks, store, cf, coeffs, bpart, counter, sums;
+    qk::DevBuf lanes, pays, psum, pm2, pitm, disc, scale, stats, greeks, store, cf, coeffs, bpart, counter, sums, mscratch;
     ~qk_ctx();
 };

@@ -35,7 +35,8 @@ __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
     if (tile >= a.n_tiles) return;                       // uniform per wavefront
     const int64_t path = tile * TILE + lid;
     const bool active = path < a.n_paths;
-    const uint32_t p0 = (uint32_t)(uint64_t)path, p1 = (uint32_t)((uint64_t)path >> 32);
+    const uint64_t gpath = (uint64_t)(a.path0 + path);     // counter of the random stream
+    const uint32_t p0 = (uint32_t)gpath, p1 = (uint32_t)(gpath >> 32);
     const int lbase = blockIdx.y * L;
     const LaneC<Real> *lc = a.lanes + lbase;
 
@@ -102,49 +103,77 @@ __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
 }
 
 // ---------------------------------------------------------------------------
-// Fixed-order merge of tile moments and FP64 finalisation.  One block per
-// (lane, payoff); thread i merges tiles i, i+256, ... then a fixed LDS tree.
+// Fixed-order merge of tile moments and FP64 finalisation, in two stages.
+// Stage 1: block (chunk, group) merges MERGE_CHUNK consecutive tiles; thread i
+// merges tiles [i*4, i*4+4) of the chunk, then a fixed LDS tree.  Stage 2: one
+// block per group merges the chunk results the same way.  The partition depends
+// only on the tile count, so the result is independent of the launch shape.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_merge(const double *__restrict__ ps, const double *__restrict__ pm2,
-                                               const uint32_t *__restrict__ pitm, int64_t n_tiles, int64_t n_paths,
-                                               int n_pay, const double *__restrict__ disc,
-                                               const double *__restrict__ scale, qk_stats *__restrict__ out) {
-    const int g = blockIdx.x;
-    const int lane = g / n_pay, j = g % n_pay;
-    const int64_t base = (int64_t)g * n_tiles;
-    double n = 0.0, mean = 0.0, m2 = 0.0;
-    unsigned long long itm = 0;
-    for (int64_t t = threadIdx.x; t < n_tiles; t += 256) {
-        const int64_t rem = n_paths - t * TILE;
-        const double nt = (double)(rem < TILE ? rem : TILE);
-        chan_merge(n, mean, m2, nt, ps[base + t] / nt, pm2[base + t]);
-        itm += pitm[base + t];
-    }
+struct Mom { double n, mean, m2; unsigned long long itm; };
+
+__device__ __forceinline__ Mom block_tree_merge(Mom m) {
     __shared__ double sn[256], smu[256], sq[256];
     __shared__ unsigned long long si[256];
-    sn[threadIdx.x] = n; smu[threadIdx.x] = mean; sq[threadIdx.x] = m2; si[threadIdx.x] = itm;
+    const int t = threadIdx.x;
+    sn[t] = m.n; smu[t] = m.mean; sq[t] = m.m2; si[t] = m.itm;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            double na = sn[threadIdx.x], ma = smu[threadIdx.x], qa = sq[threadIdx.x];
-            chan_merge(na, ma, qa, sn[threadIdx.x + s], smu[threadIdx.x + s], sq[threadIdx.x + s]);
-            sn[threadIdx.x] = na; smu[threadIdx.x] = ma; sq[threadIdx.x] = qa;
-            si[threadIdx.x] += si[threadIdx.x + s];
+        if (t < s) {
+            double na = sn[t], ma = smu[t], qa = sq[t];
+            chan_merge(na, ma, qa, sn[t + s], smu[t + s], sq[t + s]);
+            sn[t] = na; smu[t] = ma; sq[t] = qa;
+            si[t] += si[t + s];
         }
         __syncthreads();
     }
+    return Mom{sn[0], smu[0], sq[0], si[0]};
+}
+
+__global__ __launch_bounds__(256) void k_merge_chunks(const double *__restrict__ ps, const double *__restrict__ pm2,
+                                                      const uint32_t *__restrict__ pitm, int64_t n_tiles,
+                                                      int64_t n_paths, int64_t n_chunks, Mom *__restrict__ out) {
+    const int64_t g = blockIdx.y, chunk = blockIdx.x;
+    const int64_t base = g * n_tiles;
+    Mom m{0.0, 0.0, 0.0, 0ull};
+    const int64_t t0 = chunk * MERGE_CHUNK + (int64_t)threadIdx.x * (MERGE_CHUNK / 256);
+#pragma unroll
+    for (int k = 0; k < MERGE_CHUNK / 256; ++k) {
+        const int64_t t = t0 + k;
+        if (t < n_tiles) {
+            const int64_t rem = n_paths - t * TILE;
+            const double nt = (double)(rem < TILE ? rem : TILE);
+            chan_merge(m.n, m.mean, m.m2, nt, ps[base + t] / nt, pm2[base + t]);
+            m.itm += pitm[base + t];
+        }
+    }
+    m = block_tree_merge(m);
+    if (threadIdx.x == 0) out[g * n_chunks + chunk] = m;
+}
+
+__global__ __launch_bounds__(256) void k_merge_final(const Mom *__restrict__ chunks, int64_t n_chunks, int n_pay,
+                                                     const double *__restrict__ disc, const double *__restrict__ scale,
+                                                     qk_stats *__restrict__ out) {
+    const int g = blockIdx.x;
+    const int lane = g / n_pay, j = g % n_pay;
+    Mom m{0.0, 0.0, 0.0, 0ull};
+    for (int64_t c = threadIdx.x; c < n_chunks; c += 256) {
+        const Mom x = chunks[(int64_t)g * n_chunks + c];
+        chan_merge(m.n, m.mean, m.m2, x.n, x.mean, x.m2);
+        m.itm += x.itm;
+    }
+    m = block_tree_merge(m);
     if (threadIdx.x == 0) {
-        const double N = sn[0];
-        const double var = N > 1.0 ? sq[0] / (N - 1.0) : 0.0;
+        const double N = m.n;
+        const double var = N > 1.0 ? m.m2 / (N - 1.0) : 0.0;
         const double f = disc[lane] * scale[j];
         qk_stats o;
-        o.price = smu[0] * f;
+        o.price = m.mean * f;
         o.se = sqrt(var / N) * f;
-        o.prob_itm = (double)si[0] / N;
-        o.mean = smu[0];
-        o.m2 = sq[0];
+        o.prob_itm = (double)m.itm / N;
+        o.mean = m.mean;
+        o.m2 = m.m2;
         o.n = (int64_t)N;
-        o.itm = (int64_t)si[0];
+        o.itm = (int64_t)m.itm;
         out[g] = o;
     }
 }
@@ -392,11 +421,16 @@ hipError_t launch_paths(const PathArgs<Real> &a, int model, int scheme, bool pat
     return hipGetLastError();
 }
 
+int64_t merge_chunks(int64_t n_tiles) { return (n_tiles + MERGE_CHUNK - 1) / MERGE_CHUNK; }
+
 hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pitm, int64_t n_tiles, int64_t n_paths,
-                        int n_lanes, int n_pay, const double *disc, const double *scale, qk_stats *out,
+                        int n_lanes, int n_pay, const double *disc, const double *scale, void *scratch, qk_stats *out,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_merge, dim3(n_lanes * n_pay), dim3(256), 0, s, ps, pm2, pitm, n_tiles, n_paths, n_pay,
-                       disc, scale, out);
+    const int64_t nc = merge_chunks(n_tiles);
+    const int groups = n_lanes * n_pay;
+    hipLaunchKernelGGL(k_merge_chunks, dim3((unsigned)nc, (unsigned)groups), dim3(256), 0, s, ps, pm2, pitm, n_tiles,
+                       n_paths, nc, (Mom *)scratch);
+    hipLaunchKernelGGL(k_merge_final, dim3(groups), dim3(256), 0, s, (const Mom *)scratch, nc, n_pay, disc, scale, out);
     return hipGetLastError();
 }
 

@@ -31,8 +31,11 @@ struct LsmArgs {
 template <typename Real>
 hipError_t launch_paths(const PathArgs<Real> &a, int model, int scheme, bool pathdep, bool store, int L,
                         hipStream_t s);
+// Bytes of merge scratch for n_tiles tiles and `groups` (lane, payoff) pairs.
+int64_t merge_chunks(int64_t n_tiles);
+constexpr size_t MERGE_MOM_BYTES = 32;
 hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pitm, int64_t n_tiles, int64_t n_paths,
-                        int n_lanes, int n_pay, const double *disc, const double *scale, qk_stats *out,
+                        int n_lanes, int n_pay, const double *disc, const double *scale, void *scratch, qk_stats *out,
                         hipStream_t s);
 hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s);
 template <typename Real>

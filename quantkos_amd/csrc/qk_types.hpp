@@ -18,6 +18,7 @@ constexpr int WAVES_PER_BLOCK = 4;
 constexpr int BLOCK = TILE * WAVES_PER_BLOCK;
 constexpr int MAX_PAYOFFS = 16;
 constexpr int MAX_LANES = 64;
+constexpr int MERGE_CHUNK = 1024;   // tiles per first-stage merge block
 
 // Per-lane scheme constants, precomputed on the host in FP64 (the reference's
 // SDEScheme constructors, MSchemes.hpp:104-122, :199-216, :296-352, :485-491)
@@ -44,6 +45,7 @@ struct PathArgs {
     const PayD<Real> *pays;     // [n_pay]
     int32_t n_lanes, n_pay, n_steps, pad0;
     int64_t n_paths, n_tiles;
+    int64_t path0;              // global index of path 0 (random-stream counter offset)
     uint32_t key0, key1;
     Real itm_eps;
     double *part_sum;           // [(lane*n_pay + j) * n_tiles + tile]

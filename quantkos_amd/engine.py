@@ -47,6 +47,28 @@ def bumps(**kw) -> Bumps:
     return b
 
 
+def load_config(path: str):
+    """Config::Parser::parse + apply_price_scaling of a QuantKos YAML file (no device work)."""
+    L = _lib.load()
+    spec = _lib.RunSpec()
+    err = C.create_string_buffer(1024)
+    st = L.qk_config_load(str(path).encode(), C.byref(spec), err, 1024)
+    if st != 0:
+        raise QKError(st, err.value.decode())
+    return spec
+
+
+def run_config(path: str, device: int = 0, quiet: bool = True):
+    """MCDispatcher(Parser::parse(path)).launch_montecarlo() on `device`."""
+    L = _lib.load()
+    res = _lib.RunResult()
+    err = C.create_string_buffer(2048)
+    st = L.qk_run_config(str(path).encode(), device, int(quiet), C.byref(res), err, 2048)
+    if st != 0:
+        raise QKError(st, err.value.decode())
+    return res
+
+
 class Engine:
     """One qk_ctx: a device, a memory budget and a stream."""
 
@@ -105,9 +127,11 @@ class Engine:
         self._check(self.L.qk_price_american(self.h, C.byref(c), right, K, C.byref(out), ptr))
         return (out, co) if want_coeffs else out
 
-    def simulate_paths(self, c: Contract) -> np.ndarray:
+    def simulate_paths(self, c: Contract, path0: int = 0) -> np.ndarray:
+        """Spot paths [path0, path0 + c.n_paths) laid out [step][path]."""
         S = np.empty((c.n_steps, c.n_paths), dtype=np.float64)
-        self._check(self.L.qk_simulate_paths(self.h, C.byref(c), S.ctypes.data_as(C.POINTER(C.c_double))))
+        self._check(self.L.qk_simulate_paths(self.h, C.byref(c), path0,
+                                             S.ctypes.data_as(C.POINTER(C.c_double))))
         return S
 
     def timing(self) -> Timing:

@@ -1,0 +1,245 @@
+// qk_engine.cpp -- the engine layer over the C ABI (see qk_engine.hpp).
+#include "qk_engine.hpp"
+
+#include <cstdlib>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+
+#include "qk_paths_io.hpp"
+
+namespace quantkos::Engine {
+
+namespace {
+std::mutex g_mu;
+int g_device = -1;
+qk_ctx *g_ctx = nullptr;
+
+struct CtxGuard {
+    ~CtxGuard() {
+        if (g_ctx) qk_ctx_destroy(g_ctx);
+    }
+} g_guard;
+
+void check(qk_status st, const char *what) {
+    if (st != QK_OK) {
+        throw std::runtime_error(std::string(what) + " failed: " + qk_strerror(st) + ": " +
+                                 (g_ctx ? qk_last_error(g_ctx) : ""));
+    }
+}
+}  // namespace
+
+void set_device(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_ctx && g_device != device) {
+        qk_ctx_destroy(g_ctx);
+        g_ctx = nullptr;
+    }
+    g_device = device;
+}
+
+qk_ctx *device_context() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ctx) {
+        if (g_device < 0) {
+            const char *e = std::getenv("QK_DEVICE");
+            g_device = e ? std::atoi(e) : 0;
+        }
+        const qk_status st = qk_ctx_create(g_device, 0, &g_ctx);
+        if (st != QK_OK) {
+            g_ctx = nullptr;
+            throw std::runtime_error(std::string("MI355X device ") + std::to_string(g_device) +
+                                     " unavailable: " + qk_strerror(st));
+        }
+    }
+    return g_ctx;
+}
+
+qk_contract to_contract(const KC::UInputs &c) {
+    qk_contract k{};
+    k.model = c.model.id_model == KI::MathModel::Bates ? QK_MODEL_BATES : QK_MODEL_HESTON;
+    k.scheme = (int32_t)c.scheme.id_scheme;
+    k.precision = c.mc.precision == KI::Precision::FP32 ? QK_FP32 : QK_FP64;
+    k.n_steps = c.time.N_time_steps;
+    k.n_paths = c.mc.N_Paths;
+    k.seed = c.mc.rng_seed;
+    k.S0 = c.market.S0;
+    k.v0 = c.market.v0;
+    k.r = c.market.r;
+    k.q = c.market.q;
+    k.T = c.time.t_end;
+    const auto &h = c.model.active();
+    k.kappa = h.k;
+    k.theta = h.theta;
+    k.sigma = h.sigma;
+    k.rho = h.rho;
+    k.lambda_j = c.model.bates.lambda_J;
+    k.mu_j = c.model.bates.mu_J;
+    k.sigma_j = c.model.bates.sigma_J;
+    k.price_scale = c.get_price_scaling_factor();
+    return k;
+}
+
+qk_payoff to_payoff(const KC::UInputs &c) {
+    return qk_payoff{(int32_t)c.options.opt_type, c.options.opt_right == KI::OptRight::Call ? QK_CALL : QK_PUT,
+                     c.options.StrikePrice, c.options.BarrierPrice};
+}
+
+qk_bumps to_bumps(const KC::UInputs &c) {
+    qk_bumps b{};
+    b.delta_gamma = c.mc.compute_delta_et_gamma;
+    b.vega_vomma = c.mc.compute_vega_et_vomma;
+    b.vanna = c.mc.compute_vanna;
+    b.rho = c.mc.compute_rho;
+    b.theta = c.mc.compute_theta && c.time.t_end > c.mc.time_absolute_bump_size;
+    b.theta_central = 0;
+    b.rel_S = c.mc.spot_price_relative_bump_size;
+    b.abs_vol = c.mc.volatility_absolute_bump_size;
+    b.abs_r = c.mc.risk_free_rate_absolute_bump_size;
+    b.abs_T = c.mc.time_absolute_bump_size;
+    return b;
+}
+
+// --------------------------------------------------------------------------
+// Printing in the reference's formats (OptionPricer.hpp:251-290,
+// MCEngine.hpp:76-86, :104-117).
+// --------------------------------------------------------------------------
+void MCTimer::print_summary() const {
+    const char *ind = "  ";
+    std::cout << "\n\n" << ind << "========================================\n"
+              << ind << "        MC ELAPSED TIME    \n"
+              << ind << "========================================\n"
+              << ind << " Total time spent for the computations    : " << s_elapsed_total << "s\n"
+              << ind << " Time spent evaluating the option price   : " << s_elapsed_for_pricing << "s\n"
+              << ind << " Time spent evaluating the greeks         : " << s_elapsed_for_greeks << "s\n"
+              << ind << "========================================\n";
+}
+
+void MCGreeks::print_summary() const {
+    const char *ind = "  ";
+    std::cout << "\n\n" << ind << "========================================\n"
+              << ind << "        MONTE CARLO GREEKS    \n"
+              << ind << "========================================\n"
+              << ind << " Delta <-> Sensitivity to underlying spot price (S)    : " << delta << "\n"
+              << ind << " Gamma <-> Delta curvature                             : " << gamma << "\n"
+              << ind << " Vega <-> Sensitivity to volatility (v0)               : " << vega << "\n"
+              << ind << " Vomma <-> Vega curvature                              : " << vomma << "\n"
+              << ind << " Vanna <-> Cross sensitivity (spot price - volatility) : " << vanna << "\n"
+              << ind << " Rho <-> Sensitivity to interest rates (r)             : " << rho << "\n"
+              << ind << " Theta <-> Sensitivity to time decay (T)               : " << theta << "\n"
+              << ind << "========================================\n";
+}
+
+// evaluate_option_price (OptionPricer.hpp:204-249): the device has already
+// applied the discount factor (none for American) and the price scale.
+void OptionPricer::evaluate_option_price() {
+    if (config.mc.N_Paths <= 0) throw std::runtime_error("Cannot compute metrics: Invalid number of paths.");
+    metrics.option_price = stats.price;
+    metrics.standard_error = stats.se;
+    metrics.prob_itm = stats.prob_itm;
+    metrics.is_computed = true;
+}
+
+void OptionPricer::print_info_option_price() const {
+    if (!metrics.is_computed)
+        throw std::runtime_error("Attempted to print report before computing metrics. Call the method that computes "
+                                 "the payoffs first.");
+    std::ios old(nullptr);
+    old.copyfmt(std::cout);
+    std::cout << "  ========================================================\n"
+              << "                     SIMULATION RESULTS\n"
+              << "  ========================================================\n"
+              << std::fixed << std::setprecision(6)
+              << "   Estimated Option Price   :  " << metrics.option_price << "\n"
+              << "   Statistical Std Error    :  " << metrics.standard_error << "\n"
+              << "   Probability of ITM       :  " << metrics.prob_itm * 100.0 << " %\n"
+              << "  ========================================================\n"
+              << std::endl;
+    std::cout.copyfmt(old);
+}
+
+const OptionPricer::MCOpPrices &OptionPricer::get_option_price_data() const {
+    if (!metrics.is_computed) throw std::runtime_error("Metrics have not been computed yet.");
+    return metrics;
+}
+
+// --------------------------------------------------------------------------
+// Runners.
+// --------------------------------------------------------------------------
+OptionPricer::MCOpPrices price_forward(const KC::UInputs &scaled, bool print) {
+    qk_ctx *ctx = device_context();
+    const qk_contract c = to_contract(scaled);
+    const qk_payoff p = to_payoff(scaled);
+    qk_stats s{};
+    check(qk_price(ctx, &c, &p, &s), "qk_price");
+    OptionPricer pr(scaled);
+    pr.accumulate_device_stats(s);
+    pr.evaluate_option_price();
+    if (print) pr.print_info_option_price();
+    return pr.get_option_price_data();
+}
+
+OptionPricer::MCOpPrices price_american(const KC::UInputs &scaled, bool print) {
+    qk_ctx *ctx = device_context();
+    const qk_contract c = to_contract(scaled);
+    const int right = scaled.options.opt_right == KI::OptRight::Call ? QK_CALL : QK_PUT;
+    qk_stats s{};
+    check(qk_price_american(ctx, &c, right, scaled.options.StrikePrice, &s, nullptr), "qk_price_american");
+    OptionPricer pr(scaled);
+    pr.accumulate_device_stats(s);
+    pr.evaluate_option_price();
+    if (print) pr.print_info_option_price();
+    return pr.get_option_price_data();
+}
+
+// ForwardMCRunner (ForwardMC.hpp:56-135) with the Greeks as lanes of one launch.
+MCResults run_forward(KC::UInputs config) {
+    config.apply_price_scaling();
+    MCTimer timer;
+    timer.start_total();
+    timer.start_pricing();
+    const auto base = price_forward(config, true);
+    if (!config.output.filename_paths_out.empty()) write_paths_file(config);
+    timer.stop_pricing();
+    MCGreeks g{};
+    int32_t lanes = 0;
+    if (config.must_compute_greeks()) {
+        timer.start_greeks();
+        const qk_contract c = to_contract(config);
+        const qk_payoff p = to_payoff(config);
+        const qk_bumps b = to_bumps(config);
+        qk_greeks out{};
+        check(qk_price_greeks(device_context(), &c, &p, 1, &b, &out), "qk_price_greeks");
+        g = MCGreeks{out.delta, out.gamma, out.vega, out.vomma, out.vanna, out.rho, out.theta};
+        lanes = out.n_lanes;
+        timer.stop_greeks();
+        g.print_summary();
+    }
+    timer.stop_total();
+    timer.print_summary();
+    return MCResults{config, base, g, timer, lanes};
+}
+
+// ForwardBackwardMCRunner (ForwardBackwardMC.hpp:75-157): LSM pricing, Greeks
+// by the generic sequential stencil.
+MCResults run_forward_backward(KC::UInputs config) {
+    config.apply_price_scaling();
+    bool first = true;
+    MCResults r = execute_mc_framework(config, [&](const KC::UInputs &c) {
+        const bool print = first;
+        first = false;
+        return price_american(c, print);
+    });
+    if (config.must_compute_greeks()) r.Greeks.print_summary();
+    r.Timing.print_summary();
+    return r;
+}
+
+MCResults MCDispatcher::launch_montecarlo() const {
+    if (config.options.opt_type == KI::OptType::American) return run_forward_backward(config);
+    return run_forward(config);
+}
+
+}  // namespace quantkos::Engine

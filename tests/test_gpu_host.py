@@ -1,0 +1,112 @@
+"""The C++ host layer end to end on the GPU: YAML -> MCDispatcher -> kernels."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import quantkos_amd as qk
+from quantkos_amd.engine import load_config, run_config
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture
+def in_tmp(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    return tmp_path
+
+
+def test_yaml_bench_matches_published_price(in_tmp):
+    """bench_asian_option_price_no_greeks.yaml:65-67: 5.110195 +- 0.003025, ITM 54.092640 %."""
+    r = run_config(os.path.join(GOLDEN, "bench_asian_qe.yaml"))
+    combined = math.sqrt(r.stats.se ** 2 + 0.003025 ** 2)
+    assert abs(r.stats.price - 5.110195) <= 3 * combined, (r.stats.price, r.stats.se)
+    assert abs(r.stats.prob_itm - 0.5409264) < 0.005
+    assert r.t_total_s > 0 and r.greeks.n_lanes == 0
+
+
+def test_yaml_american_with_sequential_greeks(in_tmp):
+    """American put at S0 = K = 10 (Ikonen-Toivanen 0.520030) with the five Greek
+    switches: the forward-backward runner reprices every bump (MCEngine.hpp:576-735)."""
+    r = run_config(os.path.join(GOLDEN, "american_greeks.yaml"))
+    assert abs(r.stats.price - 0.520030) <= 3 * r.stats.se + max(0.02 * 0.520030, 0.005)
+    g = r.greeks
+    assert -1.0 < g.delta < 0.0 and g.gamma > 0.0 and g.vega > 0.0 and g.rho < 0.0
+    assert all(math.isfinite(x) for x in (g.delta, g.gamma, g.vega, g.vomma, g.vanna, g.rho, g.theta))
+    assert r.t_greeks_s > 0
+
+
+def read_paths(path):
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"QUANTKOS"
+    version, prec, n, t = np.frombuffer(raw[8:24], dtype=np.int32)
+    dt = float(np.frombuffer(raw[24:24 + prec], dtype=np.float32 if prec == 4 else np.float64)[0])
+    scale = float(np.frombuffer(raw[32:40], dtype=np.float64)[0])
+    data = np.frombuffer(raw[40:], dtype=np.float32 if prec == 4 else np.float64).reshape(n, t)
+    return dict(version=version, prec=prec, n=n, t=t, dt=dt, scale=scale, data=data)
+
+
+def test_paths_file_matches_device_paths(in_tmp):
+    text = open(os.path.join(GOLDEN, "bates_barrier_paths.yaml")).read()
+    (in_tmp / "cfg.yaml").write_text(text)
+    run_config(str(in_tmp / "cfg.yaml"))
+    f = read_paths(in_tmp / "qk_test_out" / "bates.paths")
+    assert (f["version"], f["prec"], f["n"], f["t"]) == (1, 8, 1000, 4)
+    assert f["dt"] == 0.25 and f["scale"] == 100.0
+    spec = load_config(str(in_tmp / "cfg.yaml"))
+    with qk.Engine(0) as e:
+        S = e.simulate_paths(spec.contract)
+    assert np.array_equal(f["data"], S.T)
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "ImplicitMilstein", "AndersonQE"])
+@pytest.mark.parametrize("model", ["Heston", "Bates"])
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+def test_no_noise_read_back(in_tmp, scheme, model, precision):
+    """NoNoiseTest.hpp:113-204: zero volatility, paths written then read back at T."""
+    jumps = "    lambda_J: 0.0\n    mu_J: 0.0\n    sigma_J: 0.0\n" if model == "Bates" else ""
+    (in_tmp / "nn.yaml").write_text(f"""Market:
+  Price: 100.
+  Variance: 0.
+  r: 0.05
+  q: 0.02
+Options:
+  OptionType: European
+  StrikePrice: 100.
+Model:
+  {model}:
+    k: 0.
+    theta: 0.
+    sigma: 0.
+    rho: 0.
+{jumps}Numerics:
+  Scheme: {scheme}
+Time:
+  T_End: 1.
+  Inp_DT: 0.0027397260273972603
+MC:
+  N_Paths: 100
+  Batch_Size: 100
+  Precision: {precision}
+Output:
+  out_dir: "nn_out"
+  Name_Paths_Out_File: "nn.paths"
+""")
+    run_config(str(in_tmp / "nn.yaml"))
+    f = read_paths(in_tmp / "nn_out" / "nn.paths")
+    ST = f["data"][:, -1].astype(np.float64) * f["scale"]
+    tol = 1e-8 if precision == "FP64" else 5e-3
+    assert np.abs(ST - 100.0 * math.exp(0.03)).max() < tol
+
+
+def test_cli_prints_reference_report(in_tmp):
+    cli = os.path.join(os.path.dirname(qk.__file__), "quantkos")
+    r = subprocess.run([cli, "--fp32", "--json", os.path.join(GOLDEN, "bench_asian_qe.yaml")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Estimated Option Price   :  5.1" in r.stdout
+    assert "Total time spent for the computations" in r.stdout
+    assert '"price": 5.1' in r.stdout
