@@ -87,7 +87,24 @@ def cpu_baseline(target_s=15.0):
             "sample": "%d paths x %d steps of the same QE workload (3 payoffs), FP64, %.1f s" % (n2, N_STEPS, el)}
 
 
+def copy_calibration(n_bytes):
+    """A plain device-to-device copy of known size (torch's copy kernel), run
+    under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale those counters
+    to bytes.  Runs in its own process: torch ships its own HIP runtime."""
+    import torch
+    a = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n_bytes, "launches": 3}))
+
+
 def main():
+    if "--copy-calibration" in sys.argv:
+        copy_calibration(int(sys.argv[sys.argv.index("--copy-calibration") + 1]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)

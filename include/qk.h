@@ -143,6 +143,49 @@ qk_status qk_price_american(qk_ctx *ctx, const qk_contract *c, int32_t right, do
  * draws the same random numbers whatever path0 and n_paths are. */
 qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, int64_t path0, double *S_out);
 
+/* ---- continuous-batching request server (a stream of pricing requests) ----
+ * The reference prices one configuration per process run (app/main.cpp:51-110);
+ * these entry points replace a loop of MCDispatcher::launch_montecarlo calls
+ * (Distpatcher.hpp:78-83) by a resident server.  Requests are admitted while
+ * their device footprint (path store for American options, tile moments)
+ * fits the server's arena; admitted requests are packed into batch launches
+ * that run while later requests arrive, up to max_inflight batches at once.
+ * A request's result does not depend on the requests it was batched with. */
+typedef struct {
+    size_t mem_budget_bytes;     /* device arena for admitted requests (0: 60% of free memory) */
+    int32_t max_inflight;        /* batch launches in flight, one stream each (0: 3)           */
+    int32_t max_lsm_streams;     /* American requests in their backward sweep at once (0: 4)   */
+    int64_t item_path_steps;     /* path-steps per work item (0: 131072)                      */
+    int32_t blocks_per_cu;       /* resident 256-thread blocks per CU of a batch launch (0: 8) */
+} qk_server_config;
+
+typedef struct {
+    qk_stats stats;
+    double latency_s;            /* submit -> result ready on the host   */
+    double queue_s;              /* submit -> admitted into a batch      */
+    qk_status status;
+} qk_result;
+
+typedef struct {
+    int64_t n_submitted, n_completed, n_batches, path_steps;
+    int64_t arena_bytes, arena_peak_bytes;
+    double busy_s;               /* wall time with at least one batch or sweep in flight */
+} qk_server_stats;
+
+typedef struct qk_server qk_server;
+
+qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, qk_server **out);
+qk_status qk_server_destroy(qk_server *srv);
+/* payoff->type QK_AMERICAN routes the request through Longstaff-Schwartz. */
+qk_status qk_server_submit(qk_server *srv, const qk_contract *c, const qk_payoff *payoff, uint64_t *ticket);
+qk_status qk_server_poll(qk_server *srv, uint64_t ticket, qk_result *out);    /* QK_ENOTREADY if pending */
+qk_status qk_server_wait(qk_server *srv, uint64_t ticket, qk_result *out);
+qk_status qk_server_stats_get(qk_server *srv, qk_server_stats *out);
+/* Replays a request stream: request i is submitted arrival_s[i] seconds after
+ * the call starts (non-decreasing), then every result is awaited. */
+qk_status qk_server_replay(qk_server *srv, const qk_contract *cs, const qk_payoff *ps, const double *arrival_s,
+                           int32_t n, qk_result *out);
+
 /* ---- host layer: QuantKos YAML configuration files ---- */
 typedef struct {
     qk_contract contract;     /* after the Normalize_prices scaling                 */

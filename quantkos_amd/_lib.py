@@ -68,6 +68,23 @@ class RunResult(C.Structure):
                 ("t_pricing_s", C.c_double), ("t_greeks_s", C.c_double)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("mem_budget_bytes", C.c_size_t), ("max_inflight", C.c_int32),
+                ("max_lsm_streams", C.c_int32), ("item_path_steps", C.c_int64),
+                ("blocks_per_cu", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("stats", Stats), ("latency_s", C.c_double), ("queue_s", C.c_double),
+                ("status", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("n_submitted", C.c_int64), ("n_completed", C.c_int64), ("n_batches", C.c_int64),
+                ("path_steps", C.c_int64), ("arena_bytes", C.c_int64),
+                ("arena_peak_bytes", C.c_int64), ("busy_s", C.c_double)]
+
+
 _lib = None
 
 
@@ -97,6 +114,13 @@ def load():
         "qk_price_greeks": ([vp, P(Contract), P(Payoff), C.c_int32, P(Bumps), P(Greeks)], st),
         "qk_price_american": ([vp, P(Contract), C.c_int32, C.c_double, P(Stats), P(C.c_double)], st),
         "qk_simulate_paths": ([vp, P(Contract), C.c_int64, P(C.c_double)], st),
+        "qk_server_create": ([vp, P(ServerConfig), P(vp)], st),
+        "qk_server_destroy": ([vp], st),
+        "qk_server_submit": ([vp, P(Contract), P(Payoff), P(C.c_uint64)], st),
+        "qk_server_poll": ([vp, C.c_uint64, P(Result)], st),
+        "qk_server_wait": ([vp, C.c_uint64, P(Result)], st),
+        "qk_server_stats_get": ([vp, P(ServerStats)], st),
+        "qk_server_replay": ([vp, P(Contract), P(Payoff), P(C.c_double), C.c_int32, P(Result)], st),
         "qk_config_load": ([C.c_char_p, P(RunSpec), C.c_char_p, C.c_size_t], st),
         "qk_run_config": ([C.c_char_p, C.c_int, C.c_int32, P(RunResult), C.c_char_p, C.c_size_t], st),
     }

@@ -27,7 +27,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-Wall",
 HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include"),
               "-I" + HOST]
 
-HIP_SOURCES = ["qk_kernels.hip", "qk_capi.hip"]
+HIP_SOURCES = ["qk_kernels.hip", "qk_capi.hip", "qk_server.hip"]
 
 
 def _run(cmd):

@@ -27,18 +27,22 @@ namespace qk {
 // stored per (lane, payoff, tile) so the merge order is fixed by the tile
 // index: results do not depend on the grid shape or on other work in flight.
 // ---------------------------------------------------------------------------
+// One tile (64 paths, one per lane) for L lanes sharing each step's Philox
+// block.  Lane l's moments for payoff j go to part_*[l*lane_stride + j*pay_stride
+// + out_index]; path p of the tile draws counter (path0 + p, step, slot).
 template <typename Real, int MODEL, int SCHEME, bool PATHDEP, bool STORE, int L>
-__global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
+__device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int n_valid, const PayD<Real> *__restrict__ pays,
+                                         int n_pay, int n_steps, int64_t n_paths, int64_t path0, uint32_t key0,
+                                         uint32_t key1, Real itm_eps, int64_t tile, Real *__restrict__ store,
+                                         double *__restrict__ part_sum, double *__restrict__ part_m2,
+                                         uint32_t *__restrict__ part_itm, int64_t out_index, int64_t lane_stride,
+                                         int64_t pay_stride) {
     using M = Math<Real>;
     const int lid = threadIdx.x & (TILE - 1);
-    const int64_t tile = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
-    if (tile >= a.n_tiles) return;                       // uniform per wavefront
     const int64_t path = tile * TILE + lid;
-    const bool active = path < a.n_paths;
-    const uint64_t gpath = (uint64_t)(a.path0 + path);     // counter of the random stream
+    const bool active = path < n_paths;
+    const uint64_t gpath = (uint64_t)(path0 + path);     // counter of the random stream
     const uint32_t p0 = (uint32_t)gpath, p1 = (uint32_t)(gpath >> 32);
-    const int lbase = blockIdx.y * L;
-    const LaneC<Real> *lc = a.lanes + lbase;
 
     Real S[L], v[L], mx[L], mn[L];
     double sm[L];
@@ -50,8 +54,8 @@ __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
         sm[l] = 0.0;
     }
 
-    for (int t = 0; t < a.n_steps; ++t) {
-        const Words w = philox(p0, p1, (uint32_t)t, 0u, a.key0, a.key1);
+    for (int t = 0; t < n_steps; ++t) {
+        const Words w = philox(p0, p1, (uint32_t)t, 0u, key0, key1);
         const Real u0 = M::uniform(w.w0), u1 = M::uniform(w.w1);
         Real sn, cs;
         M::sincos2pi(u1, sn, cs);
@@ -62,7 +66,7 @@ __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
             Real inc = heston_step<Real, SCHEME>(lc[l], z1, z2, w.w2, v[l]);
             if constexpr (MODEL == MODEL_BATES) {
                 const Real u3 = M::uniform(w.w3);
-                if (u3 >= lc[l].eml_dt) inc += bates_jump_slow<Real>(lc[l], u3, p0, p1, (uint32_t)t, a.key0, a.key1);
+                if (u3 >= lc[l].eml_dt) inc += bates_jump_slow<Real>(lc[l], u3, p0, p1, (uint32_t)t, key0, key1);
             }
             if constexpr (PATHDEP) {
                 S[l] *= M::exp(inc);
@@ -73,7 +77,7 @@ __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
                 S[l] += inc;
             }
             if constexpr (STORE) {
-                if (active) a.store[(int64_t)t * a.n_paths + path] = S[l];
+                if (active) store[(int64_t)t * n_paths + path] = S[l];
             }
         }
     }
@@ -82,24 +86,38 @@ __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
     const double n_act = (double)__popcll(amask);
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-        if (lbase + l >= a.n_lanes) break;
+        if (l >= n_valid) break;
         const Real ST = PATHDEP ? S[l] : lc[l].S0 * M::exp(S[l]);
-        for (int j = 0; j < a.n_pay; ++j) {
-            const Real x = active ? payoff_eval<Real>(a.pays[j], ST, sm[l], mx[l], mn[l], a.n_steps) : Real(0);
+        for (int j = 0; j < n_pay; ++j) {
+            const Real x = active ? payoff_eval<Real>(pays[j], ST, sm[l], mx[l], mn[l], n_steps) : Real(0);
             const double xd = (double)x;
             const double s = wave_sum(xd);
             const double mean = s / n_act;
             const double d = active ? xd - mean : 0.0;
             const double m2 = wave_sum(d * d);
-            const unsigned long long im = __ballot(active && x > a.itm_eps);
+            const unsigned long long im = __ballot(active && x > itm_eps);
             if (lid == 0) {
-                const int64_t idx = ((int64_t)(lbase + l) * a.n_pay + j) * a.n_tiles + tile;
-                a.part_sum[idx] = s;
-                a.part_m2[idx] = m2;
-                a.part_itm[idx] = (uint32_t)__popcll(im);
+                const int64_t idx = l * lane_stride + j * pay_stride + out_index;
+                part_sum[idx] = s;
+                part_m2[idx] = m2;
+                part_itm[idx] = (uint32_t)__popcll(im);
             }
         }
     }
+}
+
+// One wavefront = one tile of 64 consecutive paths; blockIdx.y selects a group
+// of L lanes.  Tile moments are stored per (lane, payoff, tile) so the merge
+// order is fixed by the tile index: results do not depend on the grid shape.
+template <typename Real, int MODEL, int SCHEME, bool PATHDEP, bool STORE, int L>
+__global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
+    const int64_t tile = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    if (tile >= a.n_tiles) return;                       // uniform per wavefront
+    const int lbase = blockIdx.y * L;
+    run_tile<Real, MODEL, SCHEME, PATHDEP, STORE, L>(
+        a.lanes + lbase, a.n_lanes - lbase, a.pays, a.n_pay, a.n_steps, a.n_paths, a.path0, a.key0, a.key1,
+        a.itm_eps, tile, a.store, a.part_sum, a.part_m2, a.part_itm, (int64_t)lbase * a.n_pay * a.n_tiles + tile,
+        (int64_t)a.n_pay * a.n_tiles, a.n_tiles);
 }
 
 // ---------------------------------------------------------------------------
@@ -390,6 +408,103 @@ __global__ __launch_bounds__(BLOCK) void k_lsm_final(LsmArgs<Real> a, double *pa
 }
 
 // ---------------------------------------------------------------------------
+// Serving: one batch of requests of uneven sizes in one launch.  Work items
+// are (request, tile range); each wavefront takes the next item from the
+// batch's counter until the list is exhausted, so every wavefront exits once
+// the items run out.  A request's tile moments land at its own offset, so its
+// result does not depend on the batch it ran in.
+// ---------------------------------------------------------------------------
+template <typename Real, int MODEL, int SCHEME>
+__device__ __forceinline__ void batch_tile(const ReqD<Real> *r, int64_t tile) {
+    const int f = r->flags;
+    if (f & REQ_STORE) {
+        run_tile<Real, MODEL, SCHEME, true, true, 1>(&r->lc, 1, &r->pay, 0, r->n_steps, r->n_paths, 0, r->key0,
+                                                     r->key1, r->itm_eps, tile, r->store, nullptr, nullptr, nullptr,
+                                                     0, 0, 0);
+    } else if (f & REQ_PATHDEP) {
+        run_tile<Real, MODEL, SCHEME, true, false, 1>(&r->lc, 1, &r->pay, 1, r->n_steps, r->n_paths, 0, r->key0,
+                                                      r->key1, r->itm_eps, tile, nullptr, r->psum, r->pm2, r->pitm,
+                                                      tile, 0, 0);
+    } else {
+        run_tile<Real, MODEL, SCHEME, false, false, 1>(&r->lc, 1, &r->pay, 1, r->n_steps, r->n_paths, 0, r->key0,
+                                                       r->key1, r->itm_eps, tile, nullptr, r->psum, r->pm2, r->pitm,
+                                                       tile, 0, 0);
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(BLOCK) void k_batch(const ReqD<Real> *__restrict__ reqs, const ItemD *__restrict__ items,
+                                                 int32_t n_items, unsigned int *counter) {
+    const int lid = threadIdx.x & 63;
+    for (;;) {
+        unsigned int v = 0;
+        if (lid == 0) v = atomicAdd(counter, 1u);
+        const int it = (int)__builtin_amdgcn_readfirstlane(v);
+        if (it >= n_items) return;
+        const ItemD item = items[it];
+        const ReqD<Real> *r = reqs + item.req;
+        const int sel = r->model * 3 + r->scheme;
+        for (int k = 0; k < item.n_tiles; ++k) {
+            const int64_t tile = item.tile0 + k;
+            switch (sel) {
+            case 0: batch_tile<Real, MODEL_HESTON, SCHEME_EULER>(r, tile); break;
+            case 1: batch_tile<Real, MODEL_HESTON, SCHEME_MILSTEIN>(r, tile); break;
+            case 2: batch_tile<Real, MODEL_HESTON, SCHEME_QE>(r, tile); break;
+            case 3: batch_tile<Real, MODEL_BATES, SCHEME_EULER>(r, tile); break;
+            case 4: batch_tile<Real, MODEL_BATES, SCHEME_MILSTEIN>(r, tile); break;
+            default: batch_tile<Real, MODEL_BATES, SCHEME_QE>(r, tile); break;
+            }
+        }
+    }
+}
+
+// Batched two-stage merge: stage 1 over a flattened (request, chunk) list,
+// stage 2 one block per request; both in fixed order within a request.
+__global__ __launch_bounds__(256) void k_bmerge_chunks(const MergeReqD *__restrict__ mr, const int32_t *__restrict__ chunk_req) {
+    const int req = chunk_req[blockIdx.x];
+    const MergeReqD m = mr[req];
+    const int64_t chunk = blockIdx.x - m.first_chunk;
+    Mom a{0.0, 0.0, 0.0, 0ull};
+    const int64_t t0 = chunk * MERGE_CHUNK + (int64_t)threadIdx.x * (MERGE_CHUNK / 256);
+#pragma unroll
+    for (int k = 0; k < MERGE_CHUNK / 256; ++k) {
+        const int64_t t = t0 + k;
+        if (t < m.n_tiles) {
+            const int64_t rem = m.n_paths - t * TILE;
+            const double nt = (double)(rem < TILE ? rem : TILE);
+            chan_merge(a.n, a.mean, a.m2, nt, m.psum[t] / nt, m.pm2[t]);
+            a.itm += m.pitm[t];
+        }
+    }
+    a = block_tree_merge(a);
+    if (threadIdx.x == 0) ((Mom *)m.scratch)[chunk] = a;
+}
+
+__global__ __launch_bounds__(256) void k_bmerge_final(const MergeReqD *__restrict__ mr, qk_stats *__restrict__ out) {
+    const MergeReqD m = mr[blockIdx.x];
+    Mom a{0.0, 0.0, 0.0, 0ull};
+    for (int64_t c = threadIdx.x; c < m.n_chunks; c += 256) {
+        const Mom x = ((const Mom *)m.scratch)[c];
+        chan_merge(a.n, a.mean, a.m2, x.n, x.mean, x.m2);
+        a.itm += x.itm;
+    }
+    a = block_tree_merge(a);
+    if (threadIdx.x == 0) {
+        const double N = a.n;
+        const double var = N > 1.0 ? a.m2 / (N - 1.0) : 0.0;
+        qk_stats o;
+        o.price = a.mean * m.factor;
+        o.se = sqrt(var / N) * m.factor;
+        o.prob_itm = (double)a.itm / N;
+        o.mean = a.mean;
+        o.m2 = a.m2;
+        o.n = (int64_t)N;
+        o.itm = (int64_t)a.itm;
+        out[blockIdx.x] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
 template <typename Real, int MODEL, int SCHEME>
@@ -454,6 +569,25 @@ hipError_t launch_lsm_final(const LsmArgs<Real> &a, double *ps, double *pm2, uin
     return hipGetLastError();
 }
 
+template <typename Real>
+hipError_t launch_batch(const ReqD<Real> *reqs, const ItemD *items, int32_t n_items, unsigned int *counter,
+                        int blocks, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch<Real>, dim3(blocks), dim3(BLOCK), 0, s, reqs, items, n_items, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_merge(const MergeReqD *mr, int n_reqs, const int32_t *chunk_req, int64_t n_chunks,
+                              qk_stats *out, hipStream_t s) {
+    if (n_reqs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bmerge_chunks, dim3((unsigned)n_chunks), dim3(256), 0, s, mr, chunk_req);
+    hipLaunchKernelGGL(k_bmerge_final, dim3(n_reqs), dim3(256), 0, s, mr, out);
+    return hipGetLastError();
+}
+
+template hipError_t launch_batch<float>(const ReqD<float> *, const ItemD *, int32_t, unsigned int *, int, hipStream_t);
+template hipError_t launch_batch<double>(const ReqD<double> *, const ItemD *, int32_t, unsigned int *, int,
+                                         hipStream_t);
 template hipError_t launch_paths<float>(const PathArgs<float> &, int, int, bool, bool, int, hipStream_t);
 template hipError_t launch_paths<double>(const PathArgs<double> &, int, int, bool, bool, int, hipStream_t);
 template hipError_t launch_lsm_sweep<float>(const LsmArgs<float> &, hipStream_t);

@@ -28,6 +28,38 @@ struct LsmArgs {
     Real K, disc;
 };
 
+// A request of a serving batch (one payoff, or a path store for LSM).
+constexpr int32_t REQ_PATHDEP = 1, REQ_STORE = 2;
+template <typename Real>
+struct ReqD {
+    LaneC<Real> lc;
+    PayD<Real> pay;
+    int32_t model, scheme, n_steps, flags;
+    int64_t n_paths, n_tiles;
+    uint32_t key0, key1;
+    Real itm_eps;
+    double *psum, *pm2;  // this request's tile moments [n_tiles]
+    uint32_t *pitm;
+    Real *store;         // [step][path] for REQ_STORE
+};
+struct ItemD {
+    int32_t req, n_tiles;
+    int64_t tile0;
+};
+struct MergeReqD {
+    const double *psum, *pm2;
+    const uint32_t *pitm;
+    void *scratch;       // [n_chunks] first-stage moments of this request
+    int64_t n_tiles, n_paths, first_chunk, n_chunks;
+    double factor;       // discount x price scale
+};
+
+template <typename Real>
+hipError_t launch_batch(const ReqD<Real> *reqs, const ItemD *items, int32_t n_items, unsigned int *counter,
+                        int blocks, hipStream_t s);
+hipError_t launch_batch_merge(const MergeReqD *mr, int n_reqs, const int32_t *chunk_req, int64_t n_chunks,
+                              qk_stats *out, hipStream_t s);
+
 template <typename Real>
 hipError_t launch_paths(const PathArgs<Real> &a, int model, int scheme, bool pathdep, bool store, int L,
                         hipStream_t s);

@@ -1,0 +1,710 @@
+// qk_server.hip -- continuous-batching request server (qk_server_* in include/qk.h).
+//
+// A scheduler thread admits pending requests in arrival order while their
+// device footprint fits a first-fit arena carved from one allocation, packs
+// them into a batch (work items = request x tile range), and launches it on a
+// free stream.  Up to max_inflight batches run at once, so requests arriving
+// while others are on the GPU start as soon as a stream and memory free up.
+// American requests generate their path store inside the batch launch, then
+// run their backward sweep on a stream of their own.  Completion is observed
+// by polling events; results are kept until the client polls or waits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "qk_capi_internal.hpp"
+#include "qk_launch.hpp"
+
+using namespace qk;
+
+namespace qk {
+namespace {
+
+using clk = std::chrono::steady_clock;
+constexpr size_t ALIGN = 256;
+size_t up(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
+
+// First-fit allocator over one device allocation, with coalescing frees.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0, peak = 0;
+    std::map<size_t, size_t> free_;   // offset -> bytes
+    hipError_t init(size_t bytes) {
+        size = up(bytes);
+        hipError_t e = hipMalloc((void **)&base, size);
+        if (e == hipSuccess) free_[0] = size;
+        return e;
+    }
+    size_t alloc(size_t bytes) {
+        bytes = up(std::max<size_t>(bytes, ALIGN));
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first, rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used += bytes;
+                peak = std::max(peak, used);
+                return off;
+            }
+        }
+        return SIZE_MAX;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = up(std::max<size_t>(bytes, ALIGN));
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+// Byte layout of one request's arena block.
+struct Layout {
+    size_t psum, pm2, pitm, scratch, store, cf, coeffs, bpart, counter, mreq, creq, stats, total;
+};
+
+struct SReq {
+    uint64_t ticket = 0;
+    qk_contract c{};
+    qk_payoff p{};
+    bool american = false;
+    clk::time_point t_submit, t_admit, t_done;
+    qk_status status = QK_OK;
+    qk_stats stats{};
+    std::string err;
+    size_t off = SIZE_MAX;
+    Layout lay{};
+    int64_t n_tiles = 0, n_chunks = 0;
+    bool done = false;
+    int astream = -1;
+    hipEvent_t ev = nullptr;
+    char *dev(size_t o) const;   // device address of a block section
+    char *base = nullptr;
+};
+char *SReq::dev(size_t o) const { return base + off + o; }
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_fwd = nullptr, ev_done = nullptr;
+    bool busy = false;
+    std::vector<SReq *> reqs, merged;
+    DevBuf dbuf;
+    char *hbuf = nullptr;
+    size_t hcap = 0;
+    qk_stats *hstats = nullptr;
+    size_t hstats_cap = 0;
+    clk::time_point t_launch;
+};
+
+struct AStream {
+    hipStream_t stream = nullptr;
+    SReq *req = nullptr;
+    char *hbuf = nullptr;   // pinned: MergeReqD + two doubles + qk_stats
+};
+
+}  // namespace
+}  // namespace qk
+
+struct qk_server {
+    qk_ctx *ctx = nullptr;
+    qk_server_config cfg{};
+    int n_cu = 256;
+    Arena arena;
+    std::vector<Slot> slots;
+    std::vector<AStream> astreams;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<SReq *> pending;
+    std::unordered_map<uint64_t, SReq *> reqs;
+    std::vector<SReq *> lsm_active;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    int64_t n_submitted = 0, n_completed = 0, n_batches = 0, path_steps = 0;
+    double busy_s = 0.0;
+    clk::time_point busy_since{};
+    int n_busy = 0;
+};
+
+namespace qk {
+namespace {
+
+Layout layout_of(const qk_contract &c, bool american, int64_t n_tiles, int64_t n_chunks) {
+    const size_t rs = c.precision == QK_FP32 ? 4 : 8;
+    Layout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += up(std::max<size_t>(bytes, 1));
+        return at;
+    };
+    L.psum = take(n_tiles * 8);
+    L.pm2 = take(n_tiles * 8);
+    L.pitm = take(n_tiles * 4);
+    L.scratch = take(n_chunks * MERGE_MOM_BYTES);
+    if (american) {
+        L.store = take((size_t)c.n_paths * c.n_steps * rs);
+        L.cf = take((size_t)c.n_paths * rs);
+        L.coeffs = take((size_t)c.n_steps * 4 * 8);
+        L.bpart = take((size_t)((c.n_paths + 255) / 256) * 16 * 8);
+        L.counter = take(256);
+        L.mreq = take(sizeof(MergeReqD) + 16);
+        L.creq = take(n_chunks * 4);
+        L.stats = take(sizeof(qk_stats));
+    }
+    L.total = o;
+    return L;
+}
+
+void busy_begin(qk_server *s) {
+    if (s->n_busy++ == 0) s->busy_since = clk::now();
+}
+void busy_end(qk_server *s) {
+    if (--s->n_busy == 0) s->busy_s += std::chrono::duration<double>(clk::now() - s->busy_since).count();
+}
+
+void complete(qk_server *s, SReq *r, qk_status st) {
+    if (r->off != SIZE_MAX) {
+        s->arena.release(r->off, r->lay.total);
+        r->off = SIZE_MAX;
+    }
+    r->status = st;
+    r->t_done = clk::now();
+    r->done = true;
+    ++s->n_completed;
+    if (st == QK_OK) s->path_steps += r->c.n_paths * (int64_t)r->c.n_steps;
+}
+
+template <typename Real>
+ReqD<Real> make_req(const SReq &r, double eps) {
+    ReqD<Real> d{};
+    d.lc = make_lane<Real>(r.c);
+    d.pay = PayD<Real>{r.p.type, r.p.right, (Real)r.p.K, (Real)r.p.B};
+    d.model = r.c.model;
+    d.scheme = r.c.scheme;
+    d.n_steps = r.c.n_steps;
+    d.flags = r.american ? (REQ_STORE | REQ_PATHDEP) : (path_dependent(r.p.type) ? REQ_PATHDEP : 0);
+    d.n_paths = r.c.n_paths;
+    d.n_tiles = r.n_tiles;
+    d.key0 = (uint32_t)r.c.seed;
+    d.key1 = (uint32_t)(r.c.seed >> 32);
+    d.itm_eps = (Real)eps;
+    d.psum = (double *)r.dev(r.lay.psum);
+    d.pm2 = (double *)r.dev(r.lay.pm2);
+    d.pitm = (uint32_t *)r.dev(r.lay.pitm);
+    d.store = r.american ? (Real *)r.dev(r.lay.store) : nullptr;
+    return d;
+}
+
+void add_items(std::vector<ItemD> &items, int req_index, const SReq &r, int64_t item_path_steps) {
+    const int64_t per_tile = (int64_t)TILE * r.c.n_steps;
+    const int64_t tpi = std::clamp<int64_t>(item_path_steps / std::max<int64_t>(per_tile, 1), 1, 1024);
+    for (int64_t t0 = 0; t0 < r.n_tiles; t0 += tpi)
+        items.push_back(ItemD{req_index, (int32_t)std::min(tpi, r.n_tiles - t0), t0});
+}
+
+MergeReqD merge_desc(const SReq &r, int64_t first_chunk, double factor) {
+    MergeReqD m{};
+    m.psum = (const double *)r.dev(r.lay.psum);
+    m.pm2 = (const double *)r.dev(r.lay.pm2);
+    m.pitm = (const uint32_t *)r.dev(r.lay.pitm);
+    m.scratch = r.dev(r.lay.scratch);
+    m.n_tiles = r.n_tiles;
+    m.n_paths = r.c.n_paths;
+    m.first_chunk = first_chunk;
+    m.n_chunks = r.n_chunks;
+    m.factor = factor;
+    return m;
+}
+
+// Appends the bytes of a vector to the staging buffer at 256-byte alignment.
+template <typename T>
+size_t stage(std::vector<char> &buf, const std::vector<T> &v) {
+    const size_t at = up(buf.size());
+    buf.resize(at + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(buf.data() + at, v.data(), v.size() * sizeof(T));
+    return at;
+}
+
+template <typename Real>
+hipError_t lsm_backward(SReq *r, hipStream_t s) {
+    LsmArgs<Real> a{};
+    a.store = (const Real *)r->dev(r->lay.store);
+    a.cf = (Real *)r->dev(r->lay.cf);
+    a.coeffs = (double *)r->dev(r->lay.coeffs);
+    a.bpart = (double *)r->dev(r->lay.bpart);
+    a.sums_out = nullptr;
+    a.counter = (unsigned int *)r->dev(r->lay.counter);
+    a.n_paths = r->c.n_paths;
+    a.n_steps = r->c.n_steps;
+    a.right = r->p.right;
+    a.K = (Real)r->p.K;
+    a.disc = (Real)std::exp(-r->c.r * (r->c.T / r->c.n_steps));
+    hipError_t e;
+    const int N = r->c.n_steps;
+    for (int t = N - 1; t >= 1; --t) {
+        a.t_upd = t;
+        a.t_gram = t - 1;
+        if ((e = launch_lsm_sweep<Real>(a, s)) != hipSuccess) return e;
+    }
+    a.t_upd = 0;
+    a.t_gram = -1;
+    return launch_lsm_final<Real>(a, (double *)r->dev(r->lay.psum), (double *)r->dev(r->lay.pm2),
+                                  (uint32_t *)r->dev(r->lay.pitm), r->n_tiles,
+                                  (Real)(r->c.precision == QK_FP32 ? 1e-6 : 1e-12), s);
+}
+
+// Starts the backward sweep of an American request on its stream once the
+// batch that generated its path store has finished.
+hipError_t start_american(qk_server *srv, SReq *r, AStream &as, hipEvent_t after_fwd) {
+    hipStream_t s = as.stream;
+    hipError_t e;
+    if ((e = hipStreamWaitEvent(s, after_fwd, 0)) != hipSuccess) return e;
+    MergeReqD m = merge_desc(*r, 0, r->c.price_scale);
+    std::memcpy(as.hbuf, &m, sizeof(m));
+    if ((e = hipMemcpyAsync(r->dev(r->lay.mreq), as.hbuf, sizeof(m), hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(r->dev(r->lay.creq), 0, r->n_chunks * 4, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(r->dev(r->lay.counter), 0, 256, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(r->dev(r->lay.coeffs), 0, (size_t)r->c.n_steps * 32, s)) != hipSuccess) return e;
+    e = r->c.precision == QK_FP32 ? lsm_backward<float>(r, s) : lsm_backward<double>(r, s);
+    if (e != hipSuccess) return e;
+    if ((e = launch_batch_merge((const MergeReqD *)r->dev(r->lay.mreq), 1, (const int32_t *)r->dev(r->lay.creq),
+                                r->n_chunks, (qk_stats *)r->dev(r->lay.stats), s)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(as.hbuf + 256, r->dev(r->lay.stats), sizeof(qk_stats), hipMemcpyDeviceToHost, s)) !=
+        hipSuccess)
+        return e;
+    if ((e = hipEventRecord(r->ev, s)) != hipSuccess) return e;
+    as.req = r;
+    r->astream = (int)(&as - srv->astreams.data());
+    srv->lsm_active.push_back(r);
+    busy_begin(srv);
+    return hipSuccess;
+}
+
+// Packs `batch` into the slot's staging buffer and launches it.
+hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
+    std::vector<ReqD<float>> r32;
+    std::vector<ReqD<double>> r64;
+    std::vector<ItemD> i32, i64;
+    std::vector<MergeReqD> mreq;
+    std::vector<int32_t> chunk_req;
+    slot.merged.clear();
+    for (SReq *r : batch) {
+        const double eps = r->c.precision == QK_FP32 ? 1e-6 : 1e-12;
+        if (r->c.precision == QK_FP32) {
+            add_items(i32, (int)r32.size(), *r, srv->cfg.item_path_steps);
+            r32.push_back(make_req<float>(*r, eps));
+        } else {
+            add_items(i64, (int)r64.size(), *r, srv->cfg.item_path_steps);
+            r64.push_back(make_req<double>(*r, eps));
+        }
+        if (!r->american) {
+            const int idx = (int)mreq.size();
+            mreq.push_back(merge_desc(*r, (int64_t)chunk_req.size(), std::exp(-r->c.r * r->c.T) *
+                                                                        (r->p.type == QK_BINARY_CASH ? 1.0 : r->c.price_scale)));
+            chunk_req.insert(chunk_req.end(), (size_t)r->n_chunks, idx);
+            slot.merged.push_back(r);
+        }
+    }
+    std::vector<char> buf;
+    buf.reserve(4096);
+    const size_t o_r32 = stage(buf, r32), o_r64 = stage(buf, r64), o_i32 = stage(buf, i32), o_i64 = stage(buf, i64);
+    const size_t o_m = stage(buf, mreq), o_c = stage(buf, chunk_req);
+    const size_t o_cnt = up(buf.size());
+    buf.resize(o_cnt + 256, 0);                              // two batch counters, zeroed by the copy
+    const size_t o_st = up(buf.size());
+    buf.resize(o_st + mreq.size() * sizeof(qk_stats) + 8);
+    hipError_t e;
+    if (slot.hcap < buf.size()) {
+        if (slot.hbuf) (void)hipHostFree(slot.hbuf);
+        slot.hcap = buf.size() * 2;
+        if ((e = hipHostMalloc((void **)&slot.hbuf, slot.hcap, hipHostMallocDefault)) != hipSuccess) {
+            slot.hbuf = nullptr;
+            slot.hcap = 0;
+            return e;
+        }
+    }
+    if (slot.dbuf.cap < buf.size()) {
+        if (slot.dbuf.reserve(srv->ctx, buf.size() * 2) != QK_OK) return hipErrorOutOfMemory;
+    }
+    if (slot.hstats_cap < mreq.size() + 1) {
+        if (slot.hstats) (void)hipHostFree(slot.hstats);
+        slot.hstats_cap = (mreq.size() + 1) * 2;
+        if ((e = hipHostMalloc((void **)&slot.hstats, slot.hstats_cap * sizeof(qk_stats), hipHostMallocDefault)) !=
+            hipSuccess) {
+            slot.hstats = nullptr;
+            slot.hstats_cap = 0;
+            return e;
+        }
+    }
+    std::memcpy(slot.hbuf, buf.data(), buf.size());
+    char *d = (char *)slot.dbuf.p;
+    hipStream_t s = slot.stream;
+    const int blocks = srv->n_cu * srv->cfg.blocks_per_cu;
+    if ((e = hipMemcpyAsync(d, slot.hbuf, o_st, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    const int b32 = (int)std::min<int64_t>(blocks, ((int64_t)i32.size() + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    const int b64 = (int)std::min<int64_t>(blocks, ((int64_t)i64.size() + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    if ((e = launch_batch<float>((const ReqD<float> *)(d + o_r32), (const ItemD *)(d + o_i32), (int32_t)i32.size(),
+                                 (unsigned int *)(d + o_cnt), b32, s)) != hipSuccess)
+        return e;
+    if ((e = launch_batch<double>((const ReqD<double> *)(d + o_r64), (const ItemD *)(d + o_i64), (int32_t)i64.size(),
+                                  (unsigned int *)(d + o_cnt + 128), b64, s)) != hipSuccess)
+        return e;
+    if ((e = hipEventRecord(slot.ev_fwd, s)) != hipSuccess) return e;
+    if (!mreq.empty()) {
+        if ((e = launch_batch_merge((const MergeReqD *)(d + o_m), (int)mreq.size(), (const int32_t *)(d + o_c),
+                                    (int64_t)chunk_req.size(), (qk_stats *)(d + o_st), s)) != hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(slot.hstats, d + o_st, mreq.size() * sizeof(qk_stats), hipMemcpyDeviceToHost, s)) !=
+            hipSuccess)
+            return e;
+    }
+    if ((e = hipEventRecord(slot.ev_done, s)) != hipSuccess) return e;
+    slot.reqs = batch;
+    slot.busy = true;
+    slot.t_launch = clk::now();
+    ++srv->n_batches;
+    busy_begin(srv);
+    return hipSuccess;
+}
+
+void finish_american(SReq *r, const qk_stats &st) {
+    r->stats = st;
+    // Immediate exercise at t0 (LSMAmerican.hpp:409-418).
+    const bool fp32 = r->c.precision == QK_FP32;
+    double S0 = r->c.S0, K = r->p.K;
+    if (fp32) {
+        S0 = (float)S0;
+        K = (float)K;
+    }
+    const double today = r->p.right == QK_CALL ? std::max(S0 - K, 0.0) : std::max(K - S0, 0.0);
+    if (today > r->stats.mean) {
+        r->stats.mean = today;
+        r->stats.m2 = 0.0;
+        r->stats.price = today * r->c.price_scale;
+        r->stats.se = 0.0;
+        r->stats.itm = today > (fp32 ? 1e-6 : 1e-12) ? r->stats.n : 0;
+        r->stats.prob_itm = (double)r->stats.itm / (double)r->stats.n;
+    }
+}
+
+// One pass of the scheduler: retire finished work, then admit and launch.
+// Returns true if anything changed.  Called with srv->mu held.
+bool schedule_once(qk_server *srv) {
+    bool progress = false;
+    for (Slot &slot : srv->slots) {
+        if (!slot.busy) continue;
+        const hipError_t q = hipEventQuery(slot.ev_done);
+        if (q == hipErrorNotReady) continue;
+        for (size_t i = 0; i < slot.merged.size(); ++i) {
+            SReq *r = slot.merged[i];
+            r->stats = slot.hstats[i];
+            complete(srv, r, q == hipSuccess ? QK_OK : QK_EDEVICE);
+        }
+        for (SReq *r : slot.reqs)
+            if (r->american && q != hipSuccess && r->astream < 0) complete(srv, r, QK_EDEVICE);
+        slot.busy = false;
+        slot.reqs.clear();
+        slot.merged.clear();
+        busy_end(srv);
+        progress = true;
+    }
+    for (size_t i = 0; i < srv->lsm_active.size();) {
+        SReq *r = srv->lsm_active[i];
+        const hipError_t q = hipEventQuery(r->ev);
+        if (q == hipErrorNotReady) {
+            ++i;
+            continue;
+        }
+        AStream &as = srv->astreams[r->astream];
+        qk_stats st;
+        std::memcpy(&st, as.hbuf + 256, sizeof(st));
+        if (q == hipSuccess) finish_american(r, st);
+        as.req = nullptr;
+        complete(srv, r, q == hipSuccess ? QK_OK : QK_EDEVICE);
+        srv->lsm_active.erase(srv->lsm_active.begin() + i);
+        busy_end(srv);
+        progress = true;
+    }
+    if (progress) srv->cv_done.notify_all();
+    // Admission in arrival order while memory, streams and a batch slot are free.
+    Slot *free_slot = nullptr;
+    for (Slot &slot : srv->slots)
+        if (!slot.busy) {
+            free_slot = &slot;
+            break;
+        }
+    if (!free_slot || srv->pending.empty()) return progress;
+    int free_as = 0;
+    for (const AStream &as : srv->astreams) free_as += as.req == nullptr;
+    std::vector<SReq *> batch;
+    while (!srv->pending.empty()) {
+        SReq *r = srv->pending.front();
+        if (r->lay.total > srv->arena.size) {
+            srv->pending.pop_front();
+            r->err = "request footprint exceeds the server arena";
+            complete(srv, r, QK_ENOMEM);
+            srv->cv_done.notify_all();
+            progress = true;
+            continue;
+        }
+        if (r->american && free_as == 0) break;
+        const size_t off = srv->arena.alloc(r->lay.total);
+        if (off == SIZE_MAX) break;
+        r->off = off;
+        r->base = srv->arena.base;
+        r->t_admit = clk::now();
+        if (r->american) --free_as;
+        batch.push_back(r);
+        srv->pending.pop_front();
+    }
+    if (batch.empty()) return progress;
+    hipError_t e = launch_slot(srv, *free_slot, batch);
+    if (e == hipSuccess) {
+        for (SReq *r : batch) {
+            if (!r->american) continue;
+            for (AStream &as : srv->astreams) {
+                if (as.req) continue;
+                if ((e = start_american(srv, r, as, free_slot->ev_fwd)) != hipSuccess) break;
+                break;
+            }
+            if (e != hipSuccess) break;
+        }
+    }
+    if (e != hipSuccess) {
+        for (SReq *r : batch)
+            if (!r->done && r->astream < 0) {
+                r->err = hipGetErrorString(e);
+                complete(srv, r, QK_EDEVICE);
+            }
+        srv->cv_done.notify_all();
+    }
+    return true;
+}
+
+void scheduler(qk_server *srv) {
+    (void)hipSetDevice(srv->ctx->device);
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+        const bool progress = schedule_once(srv);
+        const bool inflight = srv->n_busy > 0;
+        if (srv->stop && !inflight && srv->pending.empty()) break;
+        if (!progress) {
+            if (inflight) srv->cv_work.wait_for(lk, std::chrono::microseconds(50));
+            else srv->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+}
+
+void fill_result(const SReq *r, qk_result *out) {
+    out->stats = r->stats;
+    out->status = r->status;
+    out->latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+    out->queue_s = r->t_admit.time_since_epoch().count() ? std::chrono::duration<double>(r->t_admit - r->t_submit).count()
+                                                         : out->latency_s;
+}
+
+}  // namespace
+}  // namespace qk
+
+extern "C" qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, qk_server **out) {
+    if (!ctx || !out) return QK_EINVAL;
+    *out = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, QK_EDEVICE, "hipSetDevice");
+    qk_server *s = new qk_server();
+    s->ctx = ctx;
+    if (cfg) s->cfg = *cfg;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        delete s;
+        return fail(ctx, QK_EDEVICE, "hipMemGetInfo");
+    }
+    if (!s->cfg.mem_budget_bytes) s->cfg.mem_budget_bytes = (size_t)(0.6 * (double)free_b);
+    if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 3;
+    if (s->cfg.max_lsm_streams <= 0) s->cfg.max_lsm_streams = 4;
+    if (s->cfg.item_path_steps <= 0) s->cfg.item_path_steps = 131072;
+    if (s->cfg.blocks_per_cu <= 0) s->cfg.blocks_per_cu = 8;
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && n_cu > 0)
+        s->n_cu = n_cu;
+    if (s->cfg.mem_budget_bytes > free_b || s->arena.init(s->cfg.mem_budget_bytes) != hipSuccess) {
+        delete s;
+        return fail(ctx, QK_ENOMEM, "server arena allocation failed");
+    }
+    s->slots.resize(s->cfg.max_inflight);
+    for (Slot &sl : s->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev_fwd, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming) != hipSuccess) {
+            qk_server_destroy(s);
+            return fail(ctx, QK_EDEVICE, "stream creation failed");
+        }
+    }
+    s->astreams.resize(s->cfg.max_lsm_streams);
+    for (AStream &as : s->astreams) {
+        if (hipStreamCreateWithFlags(&as.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipHostMalloc((void **)&as.hbuf, 512, hipHostMallocDefault) != hipSuccess) {
+            qk_server_destroy(s);
+            return fail(ctx, QK_EDEVICE, "stream creation failed");
+        }
+    }
+    s->th = std::thread(scheduler, s);
+    *out = s;
+    return QK_OK;
+}
+
+extern "C" qk_status qk_server_destroy(qk_server *s) {
+    if (!s) return QK_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->th.joinable()) s->th.join();
+    (void)hipSetDevice(s->ctx->device);
+    for (Slot &sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.ev_fwd) (void)hipEventDestroy(sl.ev_fwd);
+        if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.hbuf) (void)hipHostFree(sl.hbuf);
+        if (sl.hstats) (void)hipHostFree(sl.hstats);
+        sl.dbuf.release();
+    }
+    for (AStream &as : s->astreams) {
+        if (as.stream) {
+            (void)hipStreamSynchronize(as.stream);
+            (void)hipStreamDestroy(as.stream);
+        }
+        if (as.hbuf) (void)hipHostFree(as.hbuf);
+    }
+    for (auto &kv : s->reqs) {
+        if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
+        delete kv.second;
+    }
+    if (s->arena.base) (void)hipFree(s->arena.base);
+    delete s;
+    return QK_OK;
+}
+
+extern "C" qk_status qk_server_submit(qk_server *s, const qk_contract *c, const qk_payoff *p, uint64_t *ticket) {
+    if (!s || !c || !p || !ticket) return QK_EINVAL;
+    qk_ctx *ctx = s->ctx;
+    std::lock_guard<std::mutex> lk(s->mu);
+    qk_status st = validate_contract(ctx, c);
+    if (st != QK_OK) return st;
+    const bool american = p->type == QK_AMERICAN;
+    if (american) {
+        if (p->right != QK_CALL && p->right != QK_PUT) return fail(ctx, QK_EINVAL, "payoff.right out of range");
+        if (!(p->K > 0.0)) return fail(ctx, QK_EINVAL, "payoff.K must be positive");
+    } else if ((st = validate_payoff(ctx, p)) != QK_OK) {
+        return st;
+    }
+    SReq *r = new SReq();
+    r->ticket = s->next_ticket++;
+    r->c = *c;
+    r->p = *p;
+    r->american = american;
+    r->n_tiles = (c->n_paths + TILE - 1) / TILE;
+    r->n_chunks = (r->n_tiles + MERGE_CHUNK - 1) / MERGE_CHUNK;
+    r->lay = layout_of(*c, american, r->n_tiles, r->n_chunks);
+    r->t_submit = clk::now();
+    if (american && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) {
+        delete r;
+        return fail(ctx, QK_EDEVICE, "hipEventCreate");
+    }
+    s->reqs[r->ticket] = r;
+    s->pending.push_back(r);
+    ++s->n_submitted;
+    *ticket = r->ticket;
+    s->cv_work.notify_one();
+    return QK_OK;
+}
+
+static qk_status take_result(qk_server *s, uint64_t ticket, qk_result *out, bool block) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return QK_ENOTFOUND;
+    SReq *r = it->second;
+    if (!r->done) {
+        if (!block) return QK_ENOTREADY;
+        s->cv_done.wait(lk, [&] { return r->done; });
+    }
+    fill_result(r, out);
+    if (r->ev) (void)hipEventDestroy(r->ev);
+    s->reqs.erase(it);
+    delete r;
+    return QK_OK;
+}
+
+extern "C" qk_status qk_server_poll(qk_server *s, uint64_t ticket, qk_result *out) {
+    if (!s || !out) return QK_EINVAL;
+    return take_result(s, ticket, out, false);
+}
+
+extern "C" qk_status qk_server_wait(qk_server *s, uint64_t ticket, qk_result *out) {
+    if (!s || !out) return QK_EINVAL;
+    return take_result(s, ticket, out, true);
+}
+
+extern "C" qk_status qk_server_stats_get(qk_server *s, qk_server_stats *out) {
+    if (!s || !out) return QK_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->n_submitted = s->n_submitted;
+    out->n_completed = s->n_completed;
+    out->n_batches = s->n_batches;
+    out->path_steps = s->path_steps;
+    out->arena_bytes = (int64_t)s->arena.size;
+    out->arena_peak_bytes = (int64_t)s->arena.peak;
+    out->busy_s = s->busy_s + (s->n_busy ? std::chrono::duration<double>(clk::now() - s->busy_since).count() : 0.0);
+    return QK_OK;
+}
+
+extern "C" qk_status qk_server_replay(qk_server *s, const qk_contract *cs, const qk_payoff *ps, const double *arrival_s,
+                                      int32_t n, qk_result *out) {
+    if (!s || !cs || !ps || !arrival_s || !out || n < 0) return QK_EINVAL;
+    std::vector<uint64_t> tickets((size_t)n, 0);
+    const auto t0 = clk::now();
+    qk_status first_err = QK_OK;
+    for (int32_t i = 0; i < n; ++i) {
+        const auto due = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(arrival_s[i]));
+        while (clk::now() < due) {
+            const auto left = due - clk::now();
+            if (left > std::chrono::microseconds(200)) std::this_thread::sleep_for(left - std::chrono::microseconds(100));
+        }
+        const qk_status st = qk_server_submit(s, &cs[i], &ps[i], &tickets[i]);
+        if (st != QK_OK) {
+            out[i] = qk_result{};
+            out[i].status = st;
+            if (first_err == QK_OK) first_err = st;
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        if (!tickets[i]) continue;
+        const qk_status st = qk_server_wait(s, tickets[i], &out[i]);
+        if (st != QK_OK && first_err == QK_OK) first_err = st;
+    }
+    return first_err;
+}

@@ -9,7 +9,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Bumps, Contract, Greeks, Payoff, Stats, Timing
+from ._lib import Bumps, Contract, Greeks, Payoff, Result, ServerConfig, ServerStats, Stats, Timing
 
 
 class QKError(RuntimeError):
@@ -141,3 +141,65 @@ class Engine:
 
     def synchronize(self):
         self._check(self.L.qk_synchronize(self.h))
+
+
+class Server:
+    """A continuous-batching request server (qk_server_*) on an Engine's device."""
+
+    def __init__(self, engine: Engine, mem_budget_bytes: int = 0, max_inflight: int = 0,
+                 max_lsm_streams: int = 0, item_path_steps: int = 0, blocks_per_cu: int = 0):
+        self.engine = engine
+        self.L = engine.L
+        cfg = ServerConfig(mem_budget_bytes, max_inflight, max_lsm_streams, item_path_steps, blocks_per_cu)
+        h = C.c_void_p()
+        engine._check(self.L.qk_server_create(engine.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.qk_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, c: Contract, p: Payoff) -> int:
+        t = C.c_uint64()
+        self.engine._check(self.L.qk_server_submit(self.h, C.byref(c), C.byref(p), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int):
+        """Result, or None while pending."""
+        r = Result()
+        st = self.L.qk_server_poll(self.h, ticket, C.byref(r))
+        if st == _lib.QK_ENOTREADY:
+            return None
+        if st != 0:
+            raise QKError(st, "qk_server_poll(%d)" % ticket)
+        return r
+
+    def wait(self, ticket: int) -> Result:
+        r = Result()
+        st = self.L.qk_server_wait(self.h, ticket, C.byref(r))
+        if st != 0:
+            raise QKError(st, "qk_server_wait(%d)" % ticket)
+        return r
+
+    def replay(self, contracts, payoffs, arrival_s) -> list:
+        n = len(contracts)
+        ca = (Contract * n)(*contracts)
+        pa = (Payoff * n)(*payoffs)
+        ta = (C.c_double * n)(*arrival_s)
+        out = (Result * n)()
+        st = self.L.qk_server_replay(self.h, ca, pa, ta, n, out)
+        if st != 0:
+            raise QKError(st, "qk_server_replay")
+        return list(out)
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        self.engine._check(self.L.qk_server_stats_get(self.h, C.byref(s)))
+        return s
