@@ -87,6 +87,63 @@ def cpu_baseline(target_s=15.0):
             "sample": "%d paths x %d steps of the same QE workload (3 payoffs), FP64, %.1f s" % (n2, N_STEPS, el)}
 
 
+def serving(engine, n_requests=4096, fp64_fraction=1.0):
+    """Config-4 request stream through the continuous-batching server (FP32),
+    against plain per-request FP64 launches of the same requests."""
+    import quantkos_amd as qk
+    from quantkos_amd import workloads
+    cs, ps, _ = workloads.config4_stream(n=n_requests, seed=2024, precision=qk.FP32)
+    total_ps = workloads.path_steps(cs)
+    mean_ps = total_ps / len(cs)
+    out = {"requests": len(cs), "path_steps": total_ps, "scheme": "Andersen QE",
+           "mix": "Euro 30/Asian 20/barrier 15/lookback 10/binary 5/American 20 %, Heston|Bates 50/50"}
+    # 1) capacity: the whole stream submitted at t=0
+    with qk.Server(engine) as srv:
+        t0 = time.perf_counter()
+        burst = srv.replay(cs, ps, [0.0] * len(cs))
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    cap = total_ps / wall
+    out["burst"] = {"wall_s": wall, "path_steps_per_s": cap, "batches": st.n_batches,
+                    "arena_peak_gb": st.arena_peak_bytes / 2 ** 30}
+    # 2) Poisson arrivals at 0.8x the measured capacity
+    rate = 0.8 * cap / mean_ps
+    cs2, ps2, arr = workloads.config4_stream(n=n_requests, seed=2024, precision=qk.FP32, rate=rate)
+    with qk.Server(engine) as srv:
+        t0 = time.perf_counter()
+        res = srv.replay(cs2, ps2, arr)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = sorted(r.latency_s for r in res)
+    q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))]
+    out["poisson"] = {"rate_per_s": rate, "wall_s": wall, "path_steps_per_s": total_ps / wall,
+                      "p50_ms": q(0.50) * 1e3, "p99_ms": q(0.99) * 1e3, "max_ms": lat[-1] * 1e3,
+                      "p50_queue_ms": sorted(r.queue_s for r in res)[len(res) // 2] * 1e3,
+                      "batches": st.n_batches, "failed": sum(r.status != 0 for r in res)}
+    # 3) plain per-request FP64 launches (the drop-in single-contract path)
+    step = max(1, int(round(1.0 / fp64_fraction)))
+    idx = list(range(0, len(cs), step))
+    err = []
+    ps64 = 0
+    t0 = time.perf_counter()
+    for i in idx:
+        c = cs[i]
+        c64 = qk.contract(**{f: getattr(c, f) for f, _ in c._fields_})
+        c64.precision = qk.FP64
+        p = ps[i]
+        s = engine.price_american(c64, p.right, p.K) if p.type == qk.AMERICAN else engine.price(c64, p)
+        ps64 += c.n_paths * c.n_steps
+        if s.se > 0:
+            err.append(abs(burst[i].stats.price - s.price) / s.se)
+    wall = time.perf_counter() - t0
+    out["fp64_plain"] = {"requests": len(idx), "wall_s": wall, "path_steps_per_s": ps64 / wall}
+    out["fp32_batched_over_fp64_plain"] = out["burst"]["path_steps_per_s"] / out["fp64_plain"]["path_steps_per_s"]
+    err.sort()
+    out["fp32_err_in_se_vs_fp64"] = {"max": err[-1], "p99": err[int(0.99 * (len(err) - 1))],
+                                     "within_1se": sum(e <= 1.0 for e in err) / len(err)}
+    return out
+
+
 def copy_calibration(n_bytes):
     """A plain device-to-device copy of known size (torch's copy kernel), run
     under `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` to scale those counters
@@ -111,6 +168,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=4096)
+    ap.add_argument("--serving-fp64-fraction", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -120,6 +180,7 @@ def main():
     r32, dt32, k32 = run(eng, qk.FP32, args.steps, args.warmup)
     s64 = args.fp64_steps or max(1, args.steps // 2)
     r64, dt64, k64 = run(eng, qk.FP64, s64, 1)
+    serve = None if args.no_serving else serving(eng, args.serving_requests, args.serving_fp64_fraction)
     eng.close()
     ps = N_PATHS * N_STEPS
     v32 = ps * args.steps / dt32
@@ -147,6 +208,8 @@ def main():
         "prices": {"fp32": [round(a.price, 6) for a in r32], "fp64": [round(b.price, 6) for b in r64],
                    "se_fp64": [round(b.se, 6) for b in r64], "fp32_err_in_se": [round(e, 4) for e in err_se]},
     }
+    if serve is not None:
+        line["serving"] = serve
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))

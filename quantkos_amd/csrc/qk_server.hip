@@ -459,18 +459,24 @@ bool schedule_once(qk_server *srv) {
     if (!free_slot || srv->pending.empty()) return progress;
     int free_as = 0;
     for (const AStream &as : srv->astreams) free_as += as.req == nullptr;
+    // Arrival order, except that an American request waiting for a free LSM
+    // stream lets the requests behind it pass.  A request that does not fit
+    // the free memory stops admission, so large requests are not starved.
     std::vector<SReq *> batch;
-    while (!srv->pending.empty()) {
-        SReq *r = srv->pending.front();
+    for (auto it = srv->pending.begin(); it != srv->pending.end();) {
+        SReq *r = *it;
         if (r->lay.total > srv->arena.size) {
-            srv->pending.pop_front();
+            it = srv->pending.erase(it);
             r->err = "request footprint exceeds the server arena";
             complete(srv, r, QK_ENOMEM);
             srv->cv_done.notify_all();
             progress = true;
             continue;
         }
-        if (r->american && free_as == 0) break;
+        if (r->american && free_as == 0) {
+            ++it;
+            continue;
+        }
         const size_t off = srv->arena.alloc(r->lay.total);
         if (off == SIZE_MAX) break;
         r->off = off;
@@ -478,7 +484,7 @@ bool schedule_once(qk_server *srv) {
         r->t_admit = clk::now();
         if (r->american) --free_as;
         batch.push_back(r);
-        srv->pending.pop_front();
+        it = srv->pending.erase(it);
     }
     if (batch.empty()) return progress;
     hipError_t e = launch_slot(srv, *free_slot, batch);
