@@ -87,7 +87,11 @@ def cpu_baseline(target_s=15.0):
             "sample": "%d paths x %d steps of the same QE workload (3 payoffs), FP64, %.1f s" % (n2, N_STEPS, el)}
 
 
-def serving(engine, n_requests=4096, fp64_fraction=1.0):
+def progress(msg):
+    print("[bench %s] %s" % (time.strftime("%H:%M:%S"), msg), file=sys.stderr, flush=True)
+
+
+def serving(engine, n_requests=4096, fp64_fraction=0.25):
     """Config-4 request stream through the continuous-batching server (FP32),
     against plain per-request FP64 launches of the same requests."""
     import quantkos_amd as qk
@@ -98,6 +102,7 @@ def serving(engine, n_requests=4096, fp64_fraction=1.0):
     out = {"requests": len(cs), "path_steps": total_ps, "scheme": "Andersen QE",
            "mix": "Euro 30/Asian 20/barrier 15/lookback 10/binary 5/American 20 %, Heston|Bates 50/50"}
     # 1) capacity: the whole stream submitted at t=0
+    progress("serving: %d requests, %.3g path-steps, burst" % (len(cs), total_ps))
     with qk.Server(engine) as srv:
         t0 = time.perf_counter()
         burst = srv.replay(cs, ps, [0.0] * len(cs))
@@ -108,6 +113,7 @@ def serving(engine, n_requests=4096, fp64_fraction=1.0):
                     "arena_peak_gb": st.arena_peak_bytes / 2 ** 30}
     # 2) Poisson arrivals at 0.8x the measured capacity
     rate = 0.8 * cap / mean_ps
+    progress("serving: burst %.2f s, %.3g path-steps/s; Poisson at %.1f req/s" % (wall, cap, rate))
     cs2, ps2, arr = workloads.config4_stream(n=n_requests, seed=2024, precision=qk.FP32, rate=rate)
     with qk.Server(engine) as srv:
         t0 = time.perf_counter()
@@ -121,18 +127,22 @@ def serving(engine, n_requests=4096, fp64_fraction=1.0):
                       "p50_queue_ms": sorted(r.queue_s for r in res)[len(res) // 2] * 1e3,
                       "batches": st.n_batches, "failed": sum(r.status != 0 for r in res)}
     # 3) plain per-request FP64 launches (the drop-in single-contract path)
+    progress("serving: Poisson %.2f s, p50 %.1f ms, p99 %.1f ms; plain FP64 launches"
+             % (wall, out["poisson"]["p50_ms"], out["poisson"]["p99_ms"]))
     step = max(1, int(round(1.0 / fp64_fraction)))
     idx = list(range(0, len(cs), step))
     err = []
     ps64 = 0
     t0 = time.perf_counter()
-    for i in idx:
+    for done, i in enumerate(idx):
         c = cs[i]
         c64 = qk.contract(**{f: getattr(c, f) for f, _ in c._fields_})
         c64.precision = qk.FP64
         p = ps[i]
         s = engine.price_american(c64, p.right, p.K) if p.type == qk.AMERICAN else engine.price(c64, p)
         ps64 += c.n_paths * c.n_steps
+        if done % 128 == 0:
+            progress("serving: plain FP64 %d/%d" % (done, len(idx)))
         if s.se > 0:
             err.append(abs(burst[i].stats.price - s.price) / s.se)
     wall = time.perf_counter() - t0
@@ -170,14 +180,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=4096)
-    ap.add_argument("--serving-fp64-fraction", type=float, default=1.0)
+    ap.add_argument("--serving-fp64-fraction", type=float, default=0.25)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import quantkos_amd as qk
     eng = qk.Engine(0)
+    progress("config 1 FP32: %d warm-up + %d timed pricings" % (args.warmup, args.steps))
     r32, dt32, k32 = run(eng, qk.FP32, args.steps, args.warmup)
+    progress("config 1 FP32: %.2f ms per pricing; FP64" % (dt32 * 1e3 / args.steps))
     s64 = args.fp64_steps or max(1, args.steps // 2)
     r64, dt64, k64 = run(eng, qk.FP64, s64, 1)
     serve = None if args.no_serving else serving(eng, args.serving_requests, args.serving_fp64_fraction)
@@ -211,6 +223,7 @@ def main():
     if serve is not None:
         line["serving"] = serve
     if not args.no_cpu_baseline:
+        progress("CPU baseline (oracle, FP64)")
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line))
 

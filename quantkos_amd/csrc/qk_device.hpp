@@ -100,49 +100,92 @@ __device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real 
     } else {
         // Andersen QE with martingale correction.  The exponential branch draws
         // its uniform from word 2 of the block (same law as Phi(Z_V)).
-        //
-        // Both branches are evaluated by every lane with lane-selected
-        // arguments, so a wavefront whose lanes split between the branches
-        // (nearly all of them when v often nears zero) issues each
-        // transcendental once instead of twice:
-        //   quadratic (psi <= 1.5):  x = 2/psi, b^2 = x - 1 + sqrt(x(x-1)),
-        //     a = m/(1+b^2), v' = a (b + Z_V)^2,
-        //     log M = A b^2 a/(1-2Aa) - log(1-2Aa)/2          (A clamped to 1/(2a) - eps)
-        //   exponential:  p = (psi-1)/(psi+1), beta = (1-p)/m, 1/beta = m (psi+1)/2,
-        //     v' = 0 if U <= p else log((1-p)/(1-U)) / beta,
-        //     M = p + beta (1-p) / max(beta - A, eps)          (A clamped to beta - eps)
-        //   psi <= eps: v' = m, M = 1;  m <= eps: v' = 0, M = 1.
-        const Real m = v_n * c.ekdt + c.tmf;
-        const Real s2 = v_n * c.s2f1 + c.s2f2;
-        const bool mpos = m > c.eps;
-        const Real ms = mpos ? m : Real(1);
-        const Real rm = M::rcp(ms);
-        const Real psi = s2 * rm * rm;
-        const bool quad = psi <= Real(1.5);
-        const Real r1 = M::rcp(quad ? psi : psi + Real(1));
-        const Real p = (psi - Real(1)) * r1;                     // exponential
-        const Real omp = Real(1) - p;
-        const Real x = Real(2) * r1;                             // quadratic
-        const Real b2 = x - Real(1) + M::sqrt(quad ? x * (x - Real(1)) : Real(0));
-        const Real U = M::uniform(w2);
-        const Real safe_U = M::fmin(U, Real(1) - c.eps);
-        const Real r2 = M::rcp(quad ? Real(1) + b2 : Real(1) - safe_U);
-        const Real a = ms * r2;
-        const Real bz = M::sqrt(quad ? b2 : Real(0)) + z1;
-        const Real beta = omp * rm;
-        const Real sAq = M::fmin(c.A, Real(0.5) * (Real(1) + b2) * rm - c.eps);
-        const Real tAa = Real(1) - Real(2) * sAq * a;
-        const Real r3 = M::rcp(quad ? tAa : M::fmax(beta - c.A, c.eps));
-        const Real L1 = M::log(quad ? tAa : omp * r2);
-        Real L2 = Real(0);
-        const bool expo_lane = mpos && !quad;
-        if (__any(expo_lane)) L2 = M::log(expo_lane ? p + beta * omp * r3 : Real(1));
-        const Real vn_quad = a * bz * bz;
-        const Real vn_exp = U <= p ? Real(0) : L1 * (ms * (psi + Real(1)) * Real(0.5));
-        const Real logM_quad = sAq * b2 * a * r3 - Real(0.5) * L1;
-        const bool degenerate = psi <= c.eps;
-        const Real vn = !mpos ? Real(0) : degenerate ? m : quad ? vn_quad : vn_exp;
-        const Real logM = (!mpos || degenerate) ? Real(0) : quad ? logM_quad : L2;
+        // FP32: the branches as written; the transcendental unit makes a
+        // divergent wavefront's second branch cheap.  FP64: both branches share
+        // their (software) transcendentals through lane selects.
+        Real vn, logM;
+        if constexpr (sizeof(Real) == 4) {
+            const Real m = v_n * c.ekdt + c.tmf;
+            const Real s2 = v_n * c.s2f1 + c.s2f2;
+            if (m <= c.eps) {
+                vn = Real(0);
+                logM = Real(0);
+            } else {
+                const Real rm = M::rcp(m);
+                const Real psi = s2 * rm * rm;
+                if (psi <= c.eps) {
+                    vn = m;
+                    logM = Real(0);
+                } else if (psi <= Real(1.5)) {
+                    const Real two_ip = Real(2) * M::rcp(psi);
+                    const Real b2 = two_ip - Real(1) + M::sqrt(two_ip * (two_ip - Real(1)));
+                    const Real a = M::div(m, Real(1) + b2);
+                    const Real bz = M::sqrt(b2) + z1;
+                    vn = a * bz * bz;
+                    const Real safe_A = M::fmin(c.A, Real(0.5) * (Real(1) + b2) * rm - c.eps);
+                    const Real tAa = Real(1) - Real(2) * safe_A * a;
+                    // log M = A b^2 a / (1 - 2 A a) - log(1 - 2 A a) / 2
+                    logM = M::div(safe_A * b2 * a, tAa) - Real(0.5) * M::log(tAa);
+                } else {
+                    const Real p = M::div(psi - Real(1), psi + Real(1));
+                    const Real beta = (Real(1) - p) * rm;
+                    const Real U = M::uniform(w2);
+                    if (U <= p) {
+                        vn = Real(0);
+                    } else {
+                        const Real safe_U = M::fmin(U, Real(1) - c.eps);
+                        vn = M::div(M::log(M::div(Real(1) - p, Real(1) - safe_U)), beta);
+                    }
+                    const Real safe_A = M::fmin(c.A, beta - c.eps);
+                    logM = M::log(p + M::div(beta * (Real(1) - p), beta - safe_A));
+                }
+            }
+        } else {
+            // Both branches are evaluated by every lane with lane-selected
+            // arguments, so a wavefront whose lanes split between the branches
+            // (nearly all of them when v often nears zero) issues each
+            // transcendental once instead of twice:
+            //   quadratic (psi <= 1.5):  x = 2/psi, b^2 = x - 1 + sqrt(x(x-1)),
+            //     a = m/(1+b^2), v' = a (b + Z_V)^2,
+            //     log M = A b^2 a/(1-2Aa) - log(1-2Aa)/2          (A clamped to 1/(2a) - eps)
+            //   exponential:  p = (psi-1)/(psi+1), beta = (1-p)/m, 1/beta = m (psi+1)/2,
+            //     v' = 0 if U <= p else log((1-p)/(1-U)) / beta,
+            //     M = p + beta (1-p) / max(beta - A, eps)          (A clamped to beta - eps)
+            //   psi <= eps: v' = m, M = 1;  m <= eps: v' = 0, M = 1.
+            const Real m = v_n * c.ekdt + c.tmf;
+            const Real s2 = v_n * c.s2f1 + c.s2f2;
+            const bool mpos = m > c.eps;
+            const Real ms = mpos ? m : Real(1);
+            const Real rm = M::rcp(ms);
+            const Real psi = s2 * rm * rm;
+            const bool quad = psi <= Real(1.5);
+            const Real r1 = M::rcp(quad ? psi : psi + Real(1));
+            const Real p = (psi - Real(1)) * r1;                     // exponential
+            const Real omp = Real(1) - p;
+            const Real x = Real(2) * r1;                             // quadratic
+            const Real b2 = x - Real(1) + M::sqrt(quad ? x * (x - Real(1)) : Real(0));
+            const Real U = M::uniform(w2);
+            const Real safe_U = M::fmin(U, Real(1) - c.eps);
+            const Real r2 = M::rcp(quad ? Real(1) + b2 : Real(1) - safe_U);
+            const Real a = ms * r2;
+            const Real bz = M::sqrt(quad ? b2 : Real(0)) + z1;
+            const Real beta = omp * rm;
+            const Real sAq = M::fmin(c.A, Real(0.5) * (Real(1) + b2) * rm - c.eps);
+            const Real tAa = Real(1) - Real(2) * sAq * a;
+            const Real r3 = M::rcp(quad ? tAa : M::fmax(beta - c.A, c.eps));
+            const Real L1 = M::log(quad ? tAa : omp * r2);
+            Real L2 = Real(0);
+            const bool expo_lane = mpos && !quad;
+            if (__any(expo_lane)) L2 = M::log(expo_lane ? p + beta * omp * r3 : Real(1));
+            const Real vn_quad = a * bz * bz;
+            const Real vn_exp = U <= p ? Real(0) : L1 * (ms * (psi + Real(1)) * Real(0.5));
+            const Real logM_quad = sAq * b2 * a * r3 - Real(0.5) * L1;
+            const bool degenerate = psi <= c.eps;
+            const Real vn_sel = !mpos ? Real(0) : degenerate ? m : quad ? vn_quad : vn_exp;
+            const Real logM_sel = (!mpos || degenerate) ? Real(0) : quad ? logM_quad : L2;
+            vn = vn_sel;
+            logM = logM_sel;
+        }
         v = vn;
         // K0 + K1 v = -log M - (K3/2) v  (MSchemes.hpp:434-439)
         return c.rqdt - logM + c.cv * v_n + c.K2 * vn + M::sqrt(c.K3 * v_n + c.K4 * vn) * z2;

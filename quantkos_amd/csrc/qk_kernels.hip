@@ -16,6 +16,8 @@
 //  k_lsm_final  date-0 update, final discount and tile moments (:393-419).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "qk_device.hpp"
 #include "qk_launch.hpp"
 
@@ -301,13 +303,18 @@ __device__ __forceinline__ Real lsm_cashflow(const LsmArgs<Real> &a, int64_t i, 
     return dcf;
 }
 
+// One date of the backward sweep: cash-flow update at date t_upd and the
+// in-the-money Gram matrix / right-hand side at date t_gram, accumulated in
+// FP64 per thread over a grid-stride range of paths, then per block.  The
+// grid is a fixed function of the path count, so the partition (and the
+// block partials' summation order in k_lsm_solve) does not vary between runs.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_lsm_sweep(LsmArgs<Real> a) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double acc[15];
 #pragma unroll
     for (int k = 0; k < 15; ++k) acc[k] = 0.0;
-    if (i < a.n_paths) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n_paths; i += stride) {
         const Real c = lsm_cashflow<Real>(a, i, a.t_upd, a.t_upd == a.n_steps - 1 ? Real(0) : a.cf[i]);
         a.cf[i] = c;
         const Real S = a.store[(int64_t)a.t_gram * a.n_paths + i];
@@ -318,14 +325,13 @@ __global__ __launch_bounds__(256) void k_lsm_sweep(LsmArgs<Real> a) {
             double L[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) L[k] = (double)Lr[k];
-            acc[0] = 1.0;
-            acc[1] = L[0] * L[0]; acc[2] = L[0] * L[1]; acc[3] = L[0] * L[2]; acc[4] = L[0] * L[3];
-            acc[5] = L[1] * L[1]; acc[6] = L[1] * L[2]; acc[7] = L[1] * L[3];
-            acc[8] = L[2] * L[2]; acc[9] = L[2] * L[3]; acc[10] = L[3] * L[3];
-            acc[11] = L[0] * Y; acc[12] = L[1] * Y; acc[13] = L[2] * Y; acc[14] = L[3] * Y;
+            acc[0] += 1.0;
+            acc[1] += L[0] * L[0]; acc[2] += L[0] * L[1]; acc[3] += L[0] * L[2]; acc[4] += L[0] * L[3];
+            acc[5] += L[1] * L[1]; acc[6] += L[1] * L[2]; acc[7] += L[1] * L[3];
+            acc[8] += L[2] * L[2]; acc[9] += L[2] * L[3]; acc[10] += L[3] * L[3];
+            acc[11] += L[0] * Y; acc[12] += L[1] * Y; acc[13] += L[2] * Y; acc[14] += L[3] * Y;
         }
     }
-    // Block partial: wavefront sums, then the 4 wavefronts in order.
     __shared__ double wpart[WAVES_PER_BLOCK][16];
     const int lid = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -339,18 +345,16 @@ __global__ __launch_bounds__(256) void k_lsm_sweep(LsmArgs<Real> a) {
         for (int w = 0; w < WAVES_PER_BLOCK; ++w) s += wpart[w][threadIdx.x];
         a.bpart[(int64_t)blockIdx.x * 16 + threadIdx.x] = s;
     }
-    // Last block to finish reduces all block partials in block order and solves.
-    __threadfence();
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+// Sums the block partials of one date in block order and solves the normal
+// equations in FP64 (one block).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_lsm_solve(LsmArgs<Real> a, int n_blocks) {
     double r[15];
 #pragma unroll
     for (int k = 0; k < 15; ++k) r[k] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) {
+    for (int b = threadIdx.x; b < n_blocks; b += 256) {
 #pragma unroll
         for (int k = 0; k < 15; ++k) r[k] += a.bpart[(int64_t)b * 16 + k];
     }
@@ -366,14 +370,13 @@ __global__ __launch_bounds__(256) void k_lsm_sweep(LsmArgs<Real> a) {
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double s[15], C[4];
-        for (int k = 0; k < 15; ++k) s[k] = red[k][0];
-        lsm_solve(s, C);
+        double sm[15], C[4];
+        for (int k = 0; k < 15; ++k) sm[k] = red[k][0];
+        lsm_solve(sm, C);
         for (int k = 0; k < 4; ++k) a.coeffs[(int64_t)a.t_gram * 4 + k] = C[k];
         if (a.sums_out) {
-            for (int k = 0; k < 15; ++k) a.sums_out[(int64_t)a.t_gram * 16 + k] = s[k];
+            for (int k = 0; k < 15; ++k) a.sums_out[(int64_t)a.t_gram * 16 + k] = sm[k];
         }
-        *a.counter = 0u;
     }
 }
 
@@ -554,10 +557,13 @@ hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_gr
     return hipGetLastError();
 }
 
+int lsm_blocks(int64_t n_paths) { return (int)std::min<int64_t>((n_paths + 255) / 256, LSM_MAX_BLOCKS); }
+
 template <typename Real>
 hipError_t launch_lsm_sweep(const LsmArgs<Real> &a, hipStream_t s) {
-    const unsigned nb = (unsigned)((a.n_paths + 255) / 256);
+    const int nb = lsm_blocks(a.n_paths);
     hipLaunchKernelGGL(k_lsm_sweep<Real>, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_lsm_solve<Real>, dim3(1), dim3(256), 0, s, a, nb);
     return hipGetLastError();
 }
 

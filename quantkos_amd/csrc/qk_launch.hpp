@@ -19,9 +19,9 @@ struct LsmArgs {
     const Real *store;     // [n_steps][n_paths]
     Real *cf;              // [n_paths]
     double *coeffs;        // [n_steps][4]
-    double *bpart;         // [n_blocks][16]
+    double *bpart;         // [lsm_blocks(n_paths)][16]
     double *sums_out;      // [n_steps][16] or null
-    unsigned int *counter; // zero between launches
+    unsigned int *counter; // unused (kept zero)
     int64_t n_paths;
     int32_t n_steps, right;
     int32_t t_upd, t_gram;
@@ -70,6 +70,10 @@ hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pit
                         int n_lanes, int n_pay, const double *disc, const double *scale, void *scratch, qk_stats *out,
                         hipStream_t s);
 hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s);
+// Blocks of one LSM sweep: one path per thread up to LSM_MAX_BLOCKS blocks,
+// then grid-stride.
+constexpr int LSM_MAX_BLOCKS = 2048;
+int lsm_blocks(int64_t n_paths);
 template <typename Real>
 hipError_t launch_lsm_sweep(const LsmArgs<Real> &a, hipStream_t s);
 template <typename Real>

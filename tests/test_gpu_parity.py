@@ -264,28 +264,63 @@ def test_readme_bench_golden(engine):
 @pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
 def test_greek_put_call_parity(engine, precision):
     """PutCallParityTest.hpp:194-277 at its own size and bumps (1M paths, dt=T/365,
-    Heston implicit Milstein), with compute_greek_tol (:120-132)."""
-    c = qe.contract(model=qk.HESTON, scheme=qk.MILSTEIN, precision=precision, n_steps=365,
-                    n_paths=1_000_000, seed=184467440737095, S0=1.0, v0=0.04, r=0.05, q=0.0, T=1.0,
-                    kappa=2.0, theta=0.04, sigma=0.3, rho=-0.7, price_scale=100.0)
+    Heston implicit Milstein), with compute_greek_tol (:120-132).
+
+    The differences are Monte Carlo estimates of zero or of known constants.
+    compute_greek_tol scales by the *price* standard error; for vega and vomma
+    the difference estimator (the sigma0-derivatives of the simulated forward)
+    has its own standard error about four times larger at this configuration
+    (the CPU oracle on the same random numbers gives 0.050 against a price SE
+    of 0.012), so those two are checked against their own standard error,
+    estimated from 16 independent sub-runs of 62,500 paths."""
+    kw = dict(model=qk.HESTON, scheme=qk.MILSTEIN, precision=precision, n_steps=365, S0=1.0, v0=0.04,
+              r=0.05, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.3, rho=-0.7, price_scale=100.0)
+    c = qe.contract(n_paths=1_000_000, seed=184467440737095, **kw)
     if precision == qk.FP32:
         b = qe.bumps(rel_S=0.02, abs_vol=0.03, abs_r=0.005, abs_T=7.0 / 365.0)
     else:
         b = qe.bumps(rel_S=0.001, abs_vol=0.005, abs_r=0.0001, abs_T=1.0 / 365.0)
-    gc, gp = engine.greeks(c, [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0), qe.payoff(qk.EUROPEAN, qk.PUT, 1.0)], b)
+    cp = [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0), qe.payoff(qk.EUROPEAN, qk.PUT, 1.0)]
+    gc, gp = engine.greeks(c, cp, b)
     K, T, r = 100.0, 1.0, 0.05
     se = gc.base_se
 
     def tol(scale, mult):
         return max(0.01 * max(abs(scale), 1.0), mult * se, 1e-4)
 
+    subs = []
+    for k in range(16):
+        sc, sp = engine.greeks(qe.contract(n_paths=62_500, seed=1000 + k, **kw), cp, b)
+        subs.append((sc.vega - sp.vega, sc.vomma - sp.vomma))
+    sd = np.std(np.array(subs), axis=0, ddof=1) / 4.0       # SE of a 1M-path estimate
     assert abs((gc.delta - gp.delta) - 1.0) <= tol(1.0, 3.3)
     assert abs(gc.gamma - gp.gamma) <= tol(1.0, 3.3)
-    assert abs(gc.vega - gp.vega) <= tol(1.0, 3.3)
-    assert abs(gc.vomma - gp.vomma) <= tol(1.0, 8.0)
+    assert abs(gc.vega - gp.vega) <= max(tol(1.0, 3.3), 3.3 * sd[0]), (gc.vega - gp.vega, sd[0])
+    assert abs(gc.vomma - gp.vomma) <= max(tol(1.0, 8.0), 8.0 * sd[1]), (gc.vomma - gp.vomma, sd[1])
     assert abs(gc.vanna - gp.vanna) <= tol(1.0, 3.3)
     exp_rho = K * T * math.exp(-r * T)
     assert abs((gc.rho - gp.rho) - exp_rho) <= tol(exp_rho, 3.3)
     exp_theta = -r * K * math.exp(-r * T)
     assert abs((gc.theta - gp.theta) - exp_theta) <= tol(exp_theta, 5.0)
     assert gc.n_lanes == 12
+
+
+def test_greek_parity_is_exact_under_common_random_numbers(engine):
+    """Every lane sees the same paths, so on each path call - put = e^{-rT}(S_T - K)
+    exactly: the Greek differences equal the Greeks of the payoff S_T (a call with
+    a vanishing strike) plus those of the constant -K e^{-rT}, to rounding."""
+    c = qe.contract(model=qk.BATES, scheme=qk.QE, precision=qk.FP64, n_steps=100, n_paths=200_000,
+                    seed=5, S0=1.0, v0=0.04, r=0.05, q=0.01, T=1.0, kappa=2.0, theta=0.04, sigma=0.5,
+                    rho=-0.7, lambda_j=0.2, mu_j=-0.1, sigma_j=0.1, price_scale=100.0)
+    b = qe.bumps(rel_S=0.01, abs_vol=0.02, abs_r=0.001, abs_T=0.02)
+    K = 1.05
+    gc, gp, gf = engine.greeks(c, [qe.payoff(qk.EUROPEAN, qk.CALL, K), qe.payoff(qk.EUROPEAN, qk.PUT, K),
+                                   qe.payoff(qk.EUROPEAN, qk.CALL, 1e-12)], b)
+    Kr, T, r, h_r, h_T = K * 100.0, 1.0, 0.05, 0.001, 0.02
+    const_rho = (math.exp(-(r + h_r) * T) - math.exp(-(r - h_r) * T)) / (2 * h_r) * (-Kr)
+    const_theta = (math.exp(-r * (T - h_T)) - math.exp(-r * T)) / h_T * (-Kr)
+    for k, extra in (("delta", 0.0), ("gamma", 0.0), ("vega", 0.0), ("vomma", 0.0), ("vanna", 0.0),
+                     ("rho", const_rho), ("theta", const_theta)):
+        lhs = getattr(gc, k) - getattr(gp, k)
+        rhs = getattr(gf, k) + extra
+        assert math.isclose(lhs, rhs, rel_tol=1e-7, abs_tol=1e-7 * (1 + abs(getattr(gc, k)))), (k, lhs, rhs)
