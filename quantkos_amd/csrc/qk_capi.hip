@@ -194,7 +194,8 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
     if (american) {
         b += (size_t)c.n_paths * (size_t)c.n_steps * rs;   // path store [date][path]
         b += (size_t)c.n_paths * rs;                         // cash flows
-        b += (size_t)c.n_steps * 4 * 8 + ((size_t)(c.n_paths + 255) / 256) * 16 * 8;
+        int64_t chunk = 0;
+        b += (size_t)c.n_steps * 4 * 8 + (size_t)pump_blocks(c.n_paths, &chunk) * 16 * 8;
     }
     return b;
 }
@@ -521,10 +522,11 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     const int64_t n = c.n_paths;
     const int N = c.n_steps;
     const int64_t n_tiles = (n + TILE - 1) / TILE;
-    const int64_t n_blocks = (n + 255) / 256;
+    int64_t chunk = 0;
+    const int nb = pump_blocks(n, &chunk);
     if ((st = ctx->cf.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
     if ((st = ctx->coeffs.reserve(ctx, (size_t)N * 4 * 8)) != QK_OK) return st;
-    if ((st = ctx->bpart.reserve(ctx, (size_t)n_blocks * 16 * 8)) != QK_OK) return st;
+    if ((st = ctx->bpart.reserve(ctx, (size_t)nb * 16 * 8)) != QK_OK) return st;
     if ((st = ctx->psum.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pm2.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pitm.reserve(ctx, n_tiles * 4)) != QK_OK) return st;
@@ -537,27 +539,30 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     QK_HIP(hipMemcpyAsync(ctx->disc.p, &one, 8, hipMemcpyHostToDevice, s));
     QK_HIP(hipMemcpyAsync(ctx->scale.p, &scale, 8, hipMemcpyHostToDevice, s));
     QK_HIP(hipMemsetAsync(ctx->coeffs.p, 0, (size_t)N * 4 * 8, s));
-    LsmArgs<Real> a{};
-    a.store = (const Real *)ctx->store.p;
-    a.cf = (Real *)ctx->cf.p;
-    a.coeffs = (double *)ctx->coeffs.p;
-    a.bpart = (double *)ctx->bpart.p;
-    a.sums_out = nullptr;
-    a.counter = (unsigned int *)ctx->counter.p;
-    a.n_paths = n;
-    a.n_steps = N;
-    a.right = right;
-    a.K = (Real)K;
-    a.disc = (Real)std::exp(-c.r * (c.T / N));          // ForwardBackwardMC.hpp:209
-    for (int t = N - 1; t >= 1; --t) {                  // ForwardBackwardMC.hpp:219-242
-        a.t_upd = t;
-        a.t_gram = t - 1;
-        QK_HIP(launch_lsm_sweep<Real>(a, s));
+    PumpArgs<Real> a{};
+    PumpReq<Real> &q = a.r[0];
+    q.store = (const Real *)ctx->store.p;
+    q.cf = (Real *)ctx->cf.p;
+    q.bpart = (double *)ctx->bpart.p;
+    q.coeffs = (double *)ctx->coeffs.p;
+    q.psum = (double *)ctx->psum.p;
+    q.pm2 = (double *)ctx->pm2.p;
+    q.pitm = (uint32_t *)ctx->pitm.p;
+    q.n_paths = n;
+    q.chunk = chunk;
+    q.n_steps = N;
+    q.right = right;
+    q.nb = nb;
+    q.K = (Real)K;
+    q.disc = (Real)std::exp(-c.r * (c.T / N));           // ForwardBackwardMC.hpp:209
+    q.itm_eps = (Real)(c.precision == QK_FP32 ? 1e-6 : 1e-12);
+    a.n_req = 1;
+    a.block_start[0] = 0;
+    a.block_start[1] = nb;
+    for (int t = N - 1; t >= 0; --t) {                   // ForwardBackwardMC.hpp:219-242
+        q.t = t;
+        QK_HIP(launch_pump<Real>(a, s));
     }
-    a.t_upd = 0;
-    a.t_gram = -1;
-    QK_HIP(launch_lsm_final<Real>(a, (double *)ctx->psum.p, (double *)ctx->pm2.p, (uint32_t *)ctx->pitm.p, n_tiles,
-                                  (Real)(c.precision == QK_FP32 ? 1e-6 : 1e-12), s));
     QK_HIP(hipEventRecord(ctx->ev[3], s));
     QK_HIP(launch_merge((const double *)ctx->psum.p, (const double *)ctx->pm2.p, (const uint32_t *)ctx->pitm.p,
                         n_tiles, n, 1, 1, (const double *)ctx->disc.p, (const double *)ctx->scale.p,

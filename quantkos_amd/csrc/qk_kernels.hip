@@ -9,11 +9,11 @@
 //  k_merge      fixed-order Chan merge of tile moments + FP64 finalisation
 //               (OptionPricer.hpp:152-172, :204-249).
 //  k_greeks     FP64 central differences of lane prices (MCEngine.hpp:586-730).
-//  k_lsm_sweep  one Longstaff-Schwartz date: cash-flow update at date t and the
-//               Laguerre Gram / right-hand side at date t-1, block partials, and
-//               the last block's FP64 Cholesky solve (LSMAmerican.hpp:310-341,
-//               :433-473, :516-613).
-//  k_lsm_final  date-0 update, final discount and tile moments (:393-419).
+//  k_lsm_pump   one Longstaff-Schwartz date for every listed request: FP64
+//               Cholesky solve of the date's normal equations, cash-flow update,
+//               Laguerre Gram / right-hand side of the previous date, and at
+//               date 0 the final discount and tile moments (LSMAmerican.hpp:
+//               310-341, :393-419, :433-473, :516-613).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -284,83 +284,26 @@ __device__ void lsm_solve(const double s[15], double C[4]) {
     }
 }
 
-// New cash flow of one path at date t (initialisation when t == n_steps-1).
+// Regression step of one Longstaff-Schwartz date for every request listed in
+// the arguments, one block per request: sums the block partials of the
+// in-the-money Gram matrix / right-hand side of date t (written by the pump
+// step of date t+1) in block order and solves the 4x4 normal equations in FP64
+// into coeffs[t].  Requests at their last date have nothing to regress.
 template <typename Real>
-__device__ __forceinline__ Real lsm_cashflow(const LsmArgs<Real> &a, int64_t i, int t, Real cf_old) {
-    const Real S = a.store[(int64_t)t * a.n_paths + i];
-    const Real intrinsic = vanilla<Real>(a.right, S, a.K);
-    if (t == a.n_steps - 1) return intrinsic;
-    const Real dcf = cf_old * a.disc;
-    if (intrinsic > Real(0)) {
-        Real L[4];
-        laguerre4<Real>(Math<Real>::div(S, a.K), L);
-        const double *C = a.coeffs + (int64_t)t * 4;
-        double cont = 0.0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cont += C[k] * (double)L[k];
-        return (double)intrinsic > cont ? intrinsic : dcf;
-    }
-    return dcf;
-}
-
-// One date of the backward sweep: cash-flow update at date t_upd and the
-// in-the-money Gram matrix / right-hand side at date t_gram, accumulated in
-// FP64 per thread over a grid-stride range of paths, then per block.  The
-// grid is a fixed function of the path count, so the partition (and the
-// block partials' summation order in k_lsm_solve) does not vary between runs.
-template <typename Real>
-__global__ __launch_bounds__(256) void k_lsm_sweep(LsmArgs<Real> a) {
+__global__ __launch_bounds__(256) void k_lsm_coef(PumpArgs<Real> a) {
+    const PumpReq<Real> &q = a.r[blockIdx.x];
+    const int t = q.t;
+    if (t >= q.n_steps - 1) return;
+    __shared__ double red[15][256];
     double acc[15];
 #pragma unroll
     for (int k = 0; k < 15; ++k) acc[k] = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * 256;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n_paths; i += stride) {
-        const Real c = lsm_cashflow<Real>(a, i, a.t_upd, a.t_upd == a.n_steps - 1 ? Real(0) : a.cf[i]);
-        a.cf[i] = c;
-        const Real S = a.store[(int64_t)a.t_gram * a.n_paths + i];
-        if (vanilla<Real>(a.right, S, a.K) > Real(0)) {
-            Real Lr[4];
-            laguerre4<Real>(Math<Real>::div(S, a.K), Lr);
-            const double Y = (double)(c * a.disc);
-            double L[4];
+    for (int j = threadIdx.x; j < q.nb; j += 256) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) L[k] = (double)Lr[k];
-            acc[0] += 1.0;
-            acc[1] += L[0] * L[0]; acc[2] += L[0] * L[1]; acc[3] += L[0] * L[2]; acc[4] += L[0] * L[3];
-            acc[5] += L[1] * L[1]; acc[6] += L[1] * L[2]; acc[7] += L[1] * L[3];
-            acc[8] += L[2] * L[2]; acc[9] += L[2] * L[3]; acc[10] += L[3] * L[3];
-            acc[11] += L[0] * Y; acc[12] += L[1] * Y; acc[13] += L[2] * Y; acc[14] += L[3] * Y;
-        }
+        for (int k = 0; k < 15; ++k) acc[k] += q.bpart[(int64_t)j * 16 + k];
     }
-    __shared__ double wpart[WAVES_PER_BLOCK][16];
-    const int lid = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < 15; ++k) {
-        const double s = wave_sum(acc[k]);
-        if (lid == 0) wpart[wv][k] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < 15) {
-        double s = 0.0;
-        for (int w = 0; w < WAVES_PER_BLOCK; ++w) s += wpart[w][threadIdx.x];
-        a.bpart[(int64_t)blockIdx.x * 16 + threadIdx.x] = s;
-    }
-}
-
-// Sums the block partials of one date in block order and solves the normal
-// equations in FP64 (one block).
-template <typename Real>
-__global__ __launch_bounds__(256) void k_lsm_solve(LsmArgs<Real> a, int n_blocks) {
-    double r[15];
-#pragma unroll
-    for (int k = 0; k < 15; ++k) r[k] = 0.0;
-    for (int b = threadIdx.x; b < n_blocks; b += 256) {
-#pragma unroll
-        for (int k = 0; k < 15; ++k) r[k] += a.bpart[(int64_t)b * 16 + k];
-    }
-    __shared__ double red[15][256];
-#pragma unroll
-    for (int k = 0; k < 15; ++k) red[k][threadIdx.x] = r[k];
+    for (int k = 0; k < 15; ++k) red[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
@@ -373,40 +316,109 @@ __global__ __launch_bounds__(256) void k_lsm_solve(LsmArgs<Real> a, int n_blocks
         double sm[15], C[4];
         for (int k = 0; k < 15; ++k) sm[k] = red[k][0];
         lsm_solve(sm, C);
-        for (int k = 0; k < 4; ++k) a.coeffs[(int64_t)a.t_gram * 4 + k] = C[k];
-        if (a.sums_out) {
-            for (int k = 0; k < 15; ++k) a.sums_out[(int64_t)a.t_gram * 16 + k] = sm[k];
-        }
+        for (int k = 0; k < 4; ++k) q.coeffs[(int64_t)t * 4 + k] = C[k];
     }
 }
 
-// Date-0 update (or the initialisation when n_steps == 1), final one-step
-// discount, and the tile moments of the discounted cash flows.
+// Path step of one Longstaff-Schwartz date for every request listed in the
+// arguments ("pump" step).  Request q's blocks each cover `chunk` consecutive
+// paths.  At date t a block (1) updates the cash flows at date t with the
+// coefficients coeffs[t] (initialises them at the last date), (2) accumulates
+// the in-the-money Gram sums of date t-1 in FP64 into its partial, and at date
+// 0 applies the final one-step discount and writes the per-tile moments of the
+// cash flows.
 template <typename Real>
-__global__ __launch_bounds__(BLOCK) void k_lsm_final(LsmArgs<Real> a, double *part_sum, double *part_m2,
-                                                      uint32_t *part_itm, int64_t n_tiles, Real itm_eps) {
-    const int lid = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
-    if (tile >= n_tiles) return;
-    const int64_t i = tile * TILE + lid;
-    const bool active = i < a.n_paths;
-    Real x = Real(0);
-    if (active) {
-        const Real c = lsm_cashflow<Real>(a, i, a.t_upd, a.t_upd == a.n_steps - 1 ? Real(0) : a.cf[i]);
-        x = c * a.disc;
-        a.cf[i] = x;
+__global__ __launch_bounds__(256) void k_lsm_pump(PumpArgs<Real> a) {
+    int r = 0;
+    while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
+    const PumpReq<Real> &q = a.r[r];
+    const int b = (int)blockIdx.x - a.block_start[r];
+    const int t = q.t, N = q.n_steps;
+    const int64_t n = q.n_paths;
+    double C[4] = {0.0, 0.0, 0.0, 0.0};
+    if (t < N - 1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) C[k] = q.coeffs[(int64_t)t * 4 + k];
     }
-    const double n_act = (double)__popcll(__ballot(active));
-    const double xd = (double)x;
-    const double s = wave_sum(xd);
-    const double mean = s / n_act;
-    const double d = active ? xd - mean : 0.0;
-    const double m2 = wave_sum(d * d);
-    const unsigned long long im = __ballot(active && x > itm_eps);
-    if (lid == 0) {
-        part_sum[tile] = s;
-        part_m2[tile] = m2;
-        part_itm[tile] = (uint32_t)__popcll(im);
+    double g[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) g[k] = 0.0;
+    const int lid = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t p0 = (int64_t)b * q.chunk;
+    const int64_t p1 = p0 + q.chunk < n ? p0 + q.chunk : n;
+    for (int64_t base = p0; base < p1; base += 256) {
+        const int64_t i = base + threadIdx.x;
+        const bool act = i < p1;
+        Real c = Real(0);
+        if (act) {
+            const Real S = q.store[(int64_t)t * n + i];
+            const Real intrinsic = vanilla<Real>(q.right, S, q.K);
+            if (t == N - 1) {
+                c = intrinsic;
+            } else {
+                const Real dcf = q.cf[i] * q.disc;
+                c = dcf;
+                if (intrinsic > Real(0)) {
+                    Real L[4];
+                    laguerre4<Real>(Math<Real>::div(S, q.K), L);
+                    double cont = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) cont += C[k] * (double)L[k];
+                    if ((double)intrinsic > cont) c = intrinsic;
+                }
+            }
+            if (t >= 1) {
+                const Real S1 = q.store[(int64_t)(t - 1) * n + i];
+                if (vanilla<Real>(q.right, S1, q.K) > Real(0)) {
+                    Real Lr[4];
+                    laguerre4<Real>(Math<Real>::div(S1, q.K), Lr);
+                    const double Y = (double)(c * q.disc);
+                    double L[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) L[k] = (double)Lr[k];
+                    g[0] += 1.0;
+                    g[1] += L[0] * L[0]; g[2] += L[0] * L[1]; g[3] += L[0] * L[2]; g[4] += L[0] * L[3];
+                    g[5] += L[1] * L[1]; g[6] += L[1] * L[2]; g[7] += L[1] * L[3];
+                    g[8] += L[2] * L[2]; g[9] += L[2] * L[3]; g[10] += L[3] * L[3];
+                    g[11] += L[0] * Y; g[12] += L[1] * Y; g[13] += L[2] * Y; g[14] += L[3] * Y;
+                }
+            } else {
+                c = c * q.disc;   // LSMAmerican.hpp:393-405
+            }
+            q.cf[i] = c;
+        }
+        if (t == 0) {
+            const unsigned long long am = __ballot(act);
+            if (am) {
+                const double n_act = (double)__popcll(am);
+                const double xd = (double)c;
+                const double s = wave_sum(xd);
+                const double mean = s / n_act;
+                const double d = act ? xd - mean : 0.0;
+                const double m2 = wave_sum(d * d);
+                const unsigned long long im = __ballot(act && c > q.itm_eps);
+                if (lid == 0) {
+                    const int64_t tile = base / TILE + wv;
+                    q.psum[tile] = s;
+                    q.pm2[tile] = m2;
+                    q.pitm[tile] = (uint32_t)__popcll(im);
+                }
+            }
+        }
+    }
+    if (t >= 1) {
+        __shared__ double wpart[WAVES_PER_BLOCK][16];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+            const double s = wave_sum(g[k]);
+            if (lid == 0) wpart[wv][k] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < 15) {
+            double s = 0.0;
+            for (int w = 0; w < WAVES_PER_BLOCK; ++w) s += wpart[w][threadIdx.x];
+            q.bpart[(int64_t)b * 16 + threadIdx.x] = s;
+        }
     }
 }
 
@@ -557,21 +569,18 @@ hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_gr
     return hipGetLastError();
 }
 
-int lsm_blocks(int64_t n_paths) { return (int)std::min<int64_t>((n_paths + 255) / 256, LSM_MAX_BLOCKS); }
-
-template <typename Real>
-hipError_t launch_lsm_sweep(const LsmArgs<Real> &a, hipStream_t s) {
-    const int nb = lsm_blocks(a.n_paths);
-    hipLaunchKernelGGL(k_lsm_sweep<Real>, dim3(nb), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_lsm_solve<Real>, dim3(1), dim3(256), 0, s, a, nb);
-    return hipGetLastError();
+int pump_blocks(int64_t n_paths, int64_t *chunk) {
+    const int64_t nb = std::min<int64_t>((n_paths + 255) / 256, PUMP_MAX_BLOCKS);
+    const int64_t c = ((n_paths + nb - 1) / nb + 255) / 256 * 256;
+    *chunk = c;
+    return (int)((n_paths + c - 1) / c);
 }
 
 template <typename Real>
-hipError_t launch_lsm_final(const LsmArgs<Real> &a, double *ps, double *pm2, uint32_t *pitm, int64_t n_tiles,
-                            Real itm_eps, hipStream_t s) {
-    const unsigned nb = (unsigned)((n_tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
-    hipLaunchKernelGGL(k_lsm_final<Real>, dim3(nb), dim3(BLOCK), 0, s, a, ps, pm2, pitm, n_tiles, itm_eps);
+hipError_t launch_pump(const PumpArgs<Real> &a, hipStream_t s) {
+    if (a.n_req <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lsm_coef<Real>, dim3(a.n_req), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_lsm_pump<Real>, dim3(a.block_start[a.n_req]), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -596,11 +605,7 @@ template hipError_t launch_batch<double>(const ReqD<double> *, const ItemD *, in
                                          hipStream_t);
 template hipError_t launch_paths<float>(const PathArgs<float> &, int, int, bool, bool, int, hipStream_t);
 template hipError_t launch_paths<double>(const PathArgs<double> &, int, int, bool, bool, int, hipStream_t);
-template hipError_t launch_lsm_sweep<float>(const LsmArgs<float> &, hipStream_t);
-template hipError_t launch_lsm_sweep<double>(const LsmArgs<double> &, hipStream_t);
-template hipError_t launch_lsm_final<float>(const LsmArgs<float> &, double *, double *, uint32_t *, int64_t, float,
-                                            hipStream_t);
-template hipError_t launch_lsm_final<double>(const LsmArgs<double> &, double *, double *, uint32_t *, int64_t, double,
-                                             hipStream_t);
+template hipError_t launch_pump<float>(const PumpArgs<float> &, hipStream_t);
+template hipError_t launch_pump<double>(const PumpArgs<double> &, hipStream_t);
 
 }  // namespace qk

@@ -14,19 +14,29 @@ struct GreekMap {
     double h_S_real, h_vol, h_r, h_T;
 };
 
+// One request of a Longstaff-Schwartz step (k_lsm_coef + k_lsm_pump).
+constexpr int PUMP_MAX_REQ = 24;        // requests per pump launch (kernel-argument size)
+constexpr int PUMP_MAX_BLOCKS = 1024;   // blocks per request per date
 template <typename Real>
-struct LsmArgs {
-    const Real *store;     // [n_steps][n_paths]
-    Real *cf;              // [n_paths]
-    double *coeffs;        // [n_steps][4]
-    double *bpart;         // [lsm_blocks(n_paths)][16]
-    double *sums_out;      // [n_steps][16] or null
-    unsigned int *counter; // unused (kept zero)
-    int64_t n_paths;
-    int32_t n_steps, right;
-    int32_t t_upd, t_gram;
-    Real K, disc;
+struct PumpReq {
+    const Real *store;    // [n_steps][n_paths]
+    Real *cf;             // [n_paths]
+    double *bpart;        // [nb][16] Gram partials of the date below t
+    double *coeffs;       // [n_steps][4]
+    double *psum, *pm2;   // tile moments of the final cash flows
+    uint32_t *pitm;
+    int64_t n_paths, chunk;
+    int32_t n_steps, right, nb, t;   // t: the date this step processes
+    Real K, disc, itm_eps, pad;
 };
+template <typename Real>
+struct PumpArgs {
+    PumpReq<Real> r[PUMP_MAX_REQ];
+    int32_t block_start[PUMP_MAX_REQ + 1];
+    int32_t n_req;
+};
+// Blocks and paths per block of one request's pump work.
+int pump_blocks(int64_t n_paths, int64_t *chunk);
 
 // A request of a serving batch (one payoff, or a path store for LSM).
 constexpr int32_t REQ_PATHDEP = 1, REQ_STORE = 2;
@@ -70,14 +80,6 @@ hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pit
                         int n_lanes, int n_pay, const double *disc, const double *scale, void *scratch, qk_stats *out,
                         hipStream_t s);
 hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s);
-// Blocks of one LSM sweep: one path per thread up to LSM_MAX_BLOCKS blocks,
-// then grid-stride.
-constexpr int LSM_MAX_BLOCKS = 2048;
-int lsm_blocks(int64_t n_paths);
 template <typename Real>
-hipError_t launch_lsm_sweep(const LsmArgs<Real> &a, hipStream_t s);
-template <typename Real>
-hipError_t launch_lsm_final(const LsmArgs<Real> &a, double *ps, double *pm2, uint32_t *pitm, int64_t n_tiles,
-                            Real itm_eps, hipStream_t s);
-
+hipError_t launch_pump(const PumpArgs<Real> &a, hipStream_t s);
 }  // namespace qk

@@ -5,9 +5,10 @@
 // them into a batch (work items = request x tile range), and launches it on a
 // free stream.  Up to max_inflight batches run at once, so requests arriving
 // while others are on the GPU start as soon as a stream and memory free up.
-// American requests generate their path store inside the batch launch, then
-// run their backward sweep on a stream of their own.  Completion is observed
-// by polling events; results are kept until the client polls or waits.
+// American requests generate their path store inside a batch launch; once it
+// is ready they join the Longstaff-Schwartz pump, whose every launch advances
+// all joined requests by one exercise date.  Completion is observed by polling
+// events; results are kept until the client polls or waits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +34,8 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 constexpr size_t ALIGN = 256;
+constexpr int PUMP_DEPTH = 8;          // pump steps queued ahead on the pump stream
+constexpr size_t PUMP_SLOT_BYTES = 512; // pinned staging per pump slot: MergeReqD, then qk_stats at +256
 size_t up(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
 
 // First-fit allocator over one device allocation, with coalescing frees.
@@ -81,7 +84,7 @@ struct Arena {
 
 // Byte layout of one request's arena block.
 struct Layout {
-    size_t psum, pm2, pitm, scratch, store, cf, coeffs, bpart, counter, mreq, creq, stats, total;
+    size_t psum, pm2, pitm, scratch, store, cf, coeffs, bpart, mreq, creq, stats, total;
 };
 
 struct SReq {
@@ -90,23 +93,27 @@ struct SReq {
     qk_payoff p{};
     bool american = false;
     clk::time_point t_submit, t_admit, t_done;
+    bool admitted = false;
     qk_status status = QK_OK;
     qk_stats stats{};
     std::string err;
     size_t off = SIZE_MAX;
     Layout lay{};
-    int64_t n_tiles = 0, n_chunks = 0;
+    int64_t n_tiles = 0, n_chunks = 0, chunk = 0;
+    int nb = 0;
     bool done = false;
-    int astream = -1;
-    hipEvent_t ev = nullptr;
-    char *dev(size_t o) const;   // device address of a block section
+    // American: pump state
+    int t = -1;                    // next date to process
+    int pslot = -1;                // pinned staging slot while in the pump
+    hipEvent_t ev = nullptr;       // backward sweep and merge finished
+    hipEvent_t ev_fwd = nullptr;   // path store generated
     char *base = nullptr;
+    char *dev(size_t o) const { return base + off + o; }
 };
-char *SReq::dev(size_t o) const { return base + off + o; }
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t ev_fwd = nullptr, ev_done = nullptr;
+    hipEvent_t ev_done = nullptr;
     bool busy = false;
     std::vector<SReq *> reqs, merged;
     DevBuf dbuf;
@@ -114,13 +121,6 @@ struct Slot {
     size_t hcap = 0;
     qk_stats *hstats = nullptr;
     size_t hstats_cap = 0;
-    clk::time_point t_launch;
-};
-
-struct AStream {
-    hipStream_t stream = nullptr;
-    SReq *req = nullptr;
-    char *hbuf = nullptr;   // pinned: MergeReqD + two doubles + qk_stats
 };
 
 }  // namespace
@@ -132,13 +132,20 @@ struct qk_server {
     int n_cu = 256;
     Arena arena;
     std::vector<Slot> slots;
-    std::vector<AStream> astreams;
+    // Longstaff-Schwartz pump
+    hipStream_t pump_stream = nullptr;
+    char *pump_hbuf = nullptr;                 // [PUMP_MAX_REQ][PUMP_SLOT_BYTES] pinned
+    std::vector<bool> pslot_used;
+    std::deque<SReq *> lsm_waiting;            // forward launched, not yet in the pump
+    std::vector<SReq *> pump_active;           // in the pump (t >= 0)
+    std::vector<SReq *> lsm_finishing;         // merge enqueued, completion pending
+    std::deque<hipEvent_t> pump_steps;         // events of queued pump steps
+    std::vector<hipEvent_t> event_pool;
     std::thread th;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<SReq *> pending;
     std::unordered_map<uint64_t, SReq *> reqs;
-    std::vector<SReq *> lsm_active;
     uint64_t next_ticket = 1;
     bool stop = false;
     int64_t n_submitted = 0, n_completed = 0, n_batches = 0, path_steps = 0;
@@ -150,7 +157,7 @@ struct qk_server {
 namespace qk {
 namespace {
 
-Layout layout_of(const qk_contract &c, bool american, int64_t n_tiles, int64_t n_chunks) {
+Layout layout_of(const qk_contract &c, bool american, int64_t n_tiles, int64_t n_chunks, int nb) {
     const size_t rs = c.precision == QK_FP32 ? 4 : 8;
     Layout L{};
     size_t o = 0;
@@ -167,9 +174,8 @@ Layout layout_of(const qk_contract &c, bool american, int64_t n_tiles, int64_t n
         L.store = take((size_t)c.n_paths * c.n_steps * rs);
         L.cf = take((size_t)c.n_paths * rs);
         L.coeffs = take((size_t)c.n_steps * 4 * 8);
-        L.bpart = take((size_t)((c.n_paths + 255) / 256) * 16 * 8);
-        L.counter = take(256);
-        L.mreq = take(sizeof(MergeReqD) + 16);
+        L.bpart = take((size_t)nb * 16 * 8);
+        L.mreq = take(sizeof(MergeReqD));
         L.creq = take(n_chunks * 4);
         L.stats = take(sizeof(qk_stats));
     }
@@ -188,6 +194,10 @@ void complete(qk_server *s, SReq *r, qk_status st) {
     if (r->off != SIZE_MAX) {
         s->arena.release(r->off, r->lay.total);
         r->off = SIZE_MAX;
+    }
+    if (r->pslot >= 0) {
+        s->pslot_used[r->pslot] = false;
+        r->pslot = -1;
     }
     r->status = st;
     r->t_done = clk::now();
@@ -247,60 +257,15 @@ size_t stage(std::vector<char> &buf, const std::vector<T> &v) {
     return at;
 }
 
-template <typename Real>
-hipError_t lsm_backward(SReq *r, hipStream_t s) {
-    LsmArgs<Real> a{};
-    a.store = (const Real *)r->dev(r->lay.store);
-    a.cf = (Real *)r->dev(r->lay.cf);
-    a.coeffs = (double *)r->dev(r->lay.coeffs);
-    a.bpart = (double *)r->dev(r->lay.bpart);
-    a.sums_out = nullptr;
-    a.counter = (unsigned int *)r->dev(r->lay.counter);
-    a.n_paths = r->c.n_paths;
-    a.n_steps = r->c.n_steps;
-    a.right = r->p.right;
-    a.K = (Real)r->p.K;
-    a.disc = (Real)std::exp(-r->c.r * (r->c.T / r->c.n_steps));
-    hipError_t e;
-    const int N = r->c.n_steps;
-    for (int t = N - 1; t >= 1; --t) {
-        a.t_upd = t;
-        a.t_gram = t - 1;
-        if ((e = launch_lsm_sweep<Real>(a, s)) != hipSuccess) return e;
+hipEvent_t take_event(qk_server *srv) {
+    if (!srv->event_pool.empty()) {
+        hipEvent_t e = srv->event_pool.back();
+        srv->event_pool.pop_back();
+        return e;
     }
-    a.t_upd = 0;
-    a.t_gram = -1;
-    return launch_lsm_final<Real>(a, (double *)r->dev(r->lay.psum), (double *)r->dev(r->lay.pm2),
-                                  (uint32_t *)r->dev(r->lay.pitm), r->n_tiles,
-                                  (Real)(r->c.precision == QK_FP32 ? 1e-6 : 1e-12), s);
-}
-
-// Starts the backward sweep of an American request on its stream once the
-// batch that generated its path store has finished.
-hipError_t start_american(qk_server *srv, SReq *r, AStream &as, hipEvent_t after_fwd) {
-    hipStream_t s = as.stream;
-    hipError_t e;
-    if ((e = hipStreamWaitEvent(s, after_fwd, 0)) != hipSuccess) return e;
-    MergeReqD m = merge_desc(*r, 0, r->c.price_scale);
-    std::memcpy(as.hbuf, &m, sizeof(m));
-    if ((e = hipMemcpyAsync(r->dev(r->lay.mreq), as.hbuf, sizeof(m), hipMemcpyHostToDevice, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(r->dev(r->lay.creq), 0, r->n_chunks * 4, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(r->dev(r->lay.counter), 0, 256, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(r->dev(r->lay.coeffs), 0, (size_t)r->c.n_steps * 32, s)) != hipSuccess) return e;
-    e = r->c.precision == QK_FP32 ? lsm_backward<float>(r, s) : lsm_backward<double>(r, s);
-    if (e != hipSuccess) return e;
-    if ((e = launch_batch_merge((const MergeReqD *)r->dev(r->lay.mreq), 1, (const int32_t *)r->dev(r->lay.creq),
-                                r->n_chunks, (qk_stats *)r->dev(r->lay.stats), s)) != hipSuccess)
-        return e;
-    if ((e = hipMemcpyAsync(as.hbuf + 256, r->dev(r->lay.stats), sizeof(qk_stats), hipMemcpyDeviceToHost, s)) !=
-        hipSuccess)
-        return e;
-    if ((e = hipEventRecord(r->ev, s)) != hipSuccess) return e;
-    as.req = r;
-    r->astream = (int)(&as - srv->astreams.data());
-    srv->lsm_active.push_back(r);
-    busy_begin(srv);
-    return hipSuccess;
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
 }
 
 // Packs `batch` into the slot's staging buffer and launches it.
@@ -322,8 +287,9 @@ hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
         }
         if (!r->american) {
             const int idx = (int)mreq.size();
-            mreq.push_back(merge_desc(*r, (int64_t)chunk_req.size(), std::exp(-r->c.r * r->c.T) *
-                                                                        (r->p.type == QK_BINARY_CASH ? 1.0 : r->c.price_scale)));
+            mreq.push_back(merge_desc(*r, (int64_t)chunk_req.size(),
+                                      std::exp(-r->c.r * r->c.T) *
+                                          (r->p.type == QK_BINARY_CASH ? 1.0 : r->c.price_scale)));
             chunk_req.insert(chunk_req.end(), (size_t)r->n_chunks, idx);
             slot.merged.push_back(r);
         }
@@ -372,7 +338,8 @@ hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
     if ((e = launch_batch<double>((const ReqD<double> *)(d + o_r64), (const ItemD *)(d + o_i64), (int32_t)i64.size(),
                                   (unsigned int *)(d + o_cnt + 128), b64, s)) != hipSuccess)
         return e;
-    if ((e = hipEventRecord(slot.ev_fwd, s)) != hipSuccess) return e;
+    for (SReq *r : batch)
+        if (r->american && (e = hipEventRecord(r->ev_fwd, s)) != hipSuccess) return e;
     if (!mreq.empty()) {
         if ((e = launch_batch_merge((const MergeReqD *)(d + o_m), (int)mreq.size(), (const int32_t *)(d + o_c),
                                     (int64_t)chunk_req.size(), (qk_stats *)(d + o_st), s)) != hipSuccess)
@@ -384,10 +351,63 @@ hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
     if ((e = hipEventRecord(slot.ev_done, s)) != hipSuccess) return e;
     slot.reqs = batch;
     slot.busy = true;
-    slot.t_launch = clk::now();
     ++srv->n_batches;
     busy_begin(srv);
     return hipSuccess;
+}
+
+template <typename Real>
+void fill_pump_req(PumpReq<Real> &q, const SReq *r) {
+    q.store = (const Real *)r->dev(r->lay.store);
+    q.cf = (Real *)r->dev(r->lay.cf);
+    q.bpart = (double *)r->dev(r->lay.bpart);
+    q.coeffs = (double *)r->dev(r->lay.coeffs);
+    q.psum = (double *)r->dev(r->lay.psum);
+    q.pm2 = (double *)r->dev(r->lay.pm2);
+    q.pitm = (uint32_t *)r->dev(r->lay.pitm);
+    q.n_paths = r->c.n_paths;
+    q.chunk = r->chunk;
+    q.n_steps = r->c.n_steps;
+    q.right = r->p.right;
+    q.nb = r->nb;
+    q.t = r->t;
+    q.K = (Real)r->p.K;
+    q.disc = (Real)std::exp(-r->c.r * (r->c.T / r->c.n_steps));   // ForwardBackwardMC.hpp:209
+    q.itm_eps = (Real)(r->c.precision == QK_FP32 ? 1e-6 : 1e-12);
+}
+
+// One launch per precision advancing every pump request of that precision by
+// one date.
+template <typename Real>
+hipError_t launch_pump_step(qk_server *srv, bool fp32) {
+    PumpArgs<Real> a{};
+    int blocks = 0;
+    for (SReq *r : srv->pump_active) {
+        if ((r->c.precision == QK_FP32) != fp32) continue;
+        fill_pump_req<Real>(a.r[a.n_req], r);
+        a.block_start[a.n_req] = blocks;
+        blocks += r->nb;
+        ++a.n_req;
+    }
+    a.block_start[a.n_req] = blocks;
+    return launch_pump<Real>(a, srv->pump_stream);
+}
+
+// Enqueues the merge of a request whose date-0 step was just queued.
+hipError_t enqueue_lsm_merge(qk_server *srv, SReq *r) {
+    hipStream_t s = srv->pump_stream;
+    char *h = srv->pump_hbuf + (size_t)r->pslot * PUMP_SLOT_BYTES;
+    const MergeReqD m = merge_desc(*r, 0, r->c.price_scale);     // no e^{-rT}: OptionPricer.hpp:211-216
+    std::memcpy(h, &m, sizeof(m));
+    hipError_t e;
+    if ((e = hipMemcpyAsync(r->dev(r->lay.mreq), h, sizeof(m), hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(r->dev(r->lay.creq), 0, r->n_chunks * 4, s)) != hipSuccess) return e;
+    if ((e = launch_batch_merge((const MergeReqD *)r->dev(r->lay.mreq), 1, (const int32_t *)r->dev(r->lay.creq),
+                                r->n_chunks, (qk_stats *)r->dev(r->lay.stats), s)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(h + 256, r->dev(r->lay.stats), sizeof(qk_stats), hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return e;
+    return hipEventRecord(r->ev, s);
 }
 
 void finish_american(SReq *r, const qk_stats &st) {
@@ -410,8 +430,75 @@ void finish_american(SReq *r, const qk_stats &st) {
     }
 }
 
-// One pass of the scheduler: retire finished work, then admit and launch.
-// Returns true if anything changed.  Called with srv->mu held.
+// Advances the pump: admits requests whose path store is ready, queues one
+// step when fewer than PUMP_DEPTH are in flight, and hands requests that
+// reached date 0 to the merge.  Returns true if anything was queued.
+bool pump(qk_server *srv) {
+    bool progress = false;
+    while (!srv->pump_steps.empty() && hipEventQuery(srv->pump_steps.front()) != hipErrorNotReady) {
+        srv->event_pool.push_back(srv->pump_steps.front());
+        srv->pump_steps.pop_front();
+    }
+    for (auto it = srv->lsm_waiting.begin(); it != srv->lsm_waiting.end();) {
+        // A request keeps its staging slot until its merge has completed, so
+        // the slots bound the pump and the finishing list together.
+        if ((int)srv->pump_active.size() >= srv->cfg.max_lsm_streams ||
+            (int)(srv->pump_active.size() + srv->lsm_finishing.size()) >= PUMP_MAX_REQ)
+            break;
+        SReq *r = *it;
+        const hipError_t q = hipEventQuery(r->ev_fwd);
+        if (q == hipErrorNotReady) {
+            ++it;
+            continue;
+        }
+        it = srv->lsm_waiting.erase(it);
+        if (q != hipSuccess) {
+            complete(srv, r, QK_EDEVICE);
+            busy_end(srv);
+            continue;
+        }
+        for (size_t k = 0; k < srv->pslot_used.size(); ++k) {
+            if (!srv->pslot_used[k]) {
+                srv->pslot_used[k] = true;
+                r->pslot = (int)k;
+                break;
+            }
+        }
+        r->t = r->c.n_steps - 1;
+        srv->pump_active.push_back(r);
+        progress = true;
+    }
+    if (srv->pump_active.empty() || (int)srv->pump_steps.size() >= PUMP_DEPTH) return progress;
+    hipError_t e = launch_pump_step<float>(srv, true);
+    if (e == hipSuccess) e = launch_pump_step<double>(srv, false);
+    std::vector<SReq *> keep;
+    for (SReq *r : srv->pump_active) {
+        if (e == hipSuccess && r->t == 0) {
+            e = enqueue_lsm_merge(srv, r);
+            if (e == hipSuccess) {
+                srv->lsm_finishing.push_back(r);
+                continue;
+            }
+        }
+        if (e != hipSuccess) {
+            r->err = hipGetErrorString(e);
+            complete(srv, r, QK_EDEVICE);
+            busy_end(srv);
+            continue;
+        }
+        --r->t;
+        keep.push_back(r);
+    }
+    srv->pump_active.swap(keep);
+    if (hipEvent_t ev = take_event(srv)) {
+        if (hipEventRecord(ev, srv->pump_stream) == hipSuccess) srv->pump_steps.push_back(ev);
+        else srv->event_pool.push_back(ev);
+    }
+    return true;
+}
+
+// One pass of the scheduler: retire finished work, run the pump, then admit
+// and launch.  Returns true if anything changed.  Called with srv->mu held.
 bool schedule_once(qk_server *srv) {
     bool progress = false;
     for (Slot &slot : srv->slots) {
@@ -423,33 +510,32 @@ bool schedule_once(qk_server *srv) {
             r->stats = slot.hstats[i];
             complete(srv, r, q == hipSuccess ? QK_OK : QK_EDEVICE);
         }
-        for (SReq *r : slot.reqs)
-            if (r->american && q != hipSuccess && r->astream < 0) complete(srv, r, QK_EDEVICE);
         slot.busy = false;
         slot.reqs.clear();
         slot.merged.clear();
         busy_end(srv);
         progress = true;
     }
-    for (size_t i = 0; i < srv->lsm_active.size();) {
-        SReq *r = srv->lsm_active[i];
+    for (size_t i = 0; i < srv->lsm_finishing.size();) {
+        SReq *r = srv->lsm_finishing[i];
         const hipError_t q = hipEventQuery(r->ev);
         if (q == hipErrorNotReady) {
             ++i;
             continue;
         }
-        AStream &as = srv->astreams[r->astream];
         qk_stats st;
-        std::memcpy(&st, as.hbuf + 256, sizeof(st));
+        std::memcpy(&st, srv->pump_hbuf + (size_t)r->pslot * PUMP_SLOT_BYTES + 256, sizeof(st));
         if (q == hipSuccess) finish_american(r, st);
-        as.req = nullptr;
         complete(srv, r, q == hipSuccess ? QK_OK : QK_EDEVICE);
-        srv->lsm_active.erase(srv->lsm_active.begin() + i);
+        srv->lsm_finishing.erase(srv->lsm_finishing.begin() + i);
         busy_end(srv);
         progress = true;
     }
+    if (pump(srv)) progress = true;
     if (progress) srv->cv_done.notify_all();
-    // Admission in arrival order while memory, streams and a batch slot are free.
+    // Admission in arrival order while memory and a batch slot are free.  A
+    // request that does not fit the free memory stops admission, so large
+    // requests are not starved by smaller ones.
     Slot *free_slot = nullptr;
     for (Slot &slot : srv->slots)
         if (!slot.busy) {
@@ -457,24 +543,15 @@ bool schedule_once(qk_server *srv) {
             break;
         }
     if (!free_slot || srv->pending.empty()) return progress;
-    int free_as = 0;
-    for (const AStream &as : srv->astreams) free_as += as.req == nullptr;
-    // Arrival order, except that an American request waiting for a free LSM
-    // stream lets the requests behind it pass.  A request that does not fit
-    // the free memory stops admission, so large requests are not starved.
     std::vector<SReq *> batch;
-    for (auto it = srv->pending.begin(); it != srv->pending.end();) {
-        SReq *r = *it;
+    while (!srv->pending.empty()) {
+        SReq *r = srv->pending.front();
         if (r->lay.total > srv->arena.size) {
-            it = srv->pending.erase(it);
+            srv->pending.pop_front();
             r->err = "request footprint exceeds the server arena";
             complete(srv, r, QK_ENOMEM);
             srv->cv_done.notify_all();
             progress = true;
-            continue;
-        }
-        if (r->american && free_as == 0) {
-            ++it;
             continue;
         }
         const size_t off = srv->arena.alloc(r->lay.total);
@@ -482,29 +559,23 @@ bool schedule_once(qk_server *srv) {
         r->off = off;
         r->base = srv->arena.base;
         r->t_admit = clk::now();
-        if (r->american) --free_as;
+        r->admitted = true;
         batch.push_back(r);
-        it = srv->pending.erase(it);
+        srv->pending.pop_front();
     }
     if (batch.empty()) return progress;
-    hipError_t e = launch_slot(srv, *free_slot, batch);
+    const hipError_t e = launch_slot(srv, *free_slot, batch);
     if (e == hipSuccess) {
         for (SReq *r : batch) {
             if (!r->american) continue;
-            for (AStream &as : srv->astreams) {
-                if (as.req) continue;
-                if ((e = start_american(srv, r, as, free_slot->ev_fwd)) != hipSuccess) break;
-                break;
-            }
-            if (e != hipSuccess) break;
+            srv->lsm_waiting.push_back(r);
+            busy_begin(srv);
         }
-    }
-    if (e != hipSuccess) {
-        for (SReq *r : batch)
-            if (!r->done && r->astream < 0) {
-                r->err = hipGetErrorString(e);
-                complete(srv, r, QK_EDEVICE);
-            }
+    } else {
+        for (SReq *r : batch) {
+            r->err = hipGetErrorString(e);
+            complete(srv, r, QK_EDEVICE);
+        }
         srv->cv_done.notify_all();
     }
     return true;
@@ -518,7 +589,7 @@ void scheduler(qk_server *srv) {
         const bool inflight = srv->n_busy > 0;
         if (srv->stop && !inflight && srv->pending.empty()) break;
         if (!progress) {
-            if (inflight) srv->cv_work.wait_for(lk, std::chrono::microseconds(50));
+            if (inflight) srv->cv_work.wait_for(lk, std::chrono::microseconds(20));
             else srv->cv_work.wait_for(lk, std::chrono::milliseconds(5));
         }
     }
@@ -528,8 +599,7 @@ void fill_result(const SReq *r, qk_result *out) {
     out->stats = r->stats;
     out->status = r->status;
     out->latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
-    out->queue_s = r->t_admit.time_since_epoch().count() ? std::chrono::duration<double>(r->t_admit - r->t_submit).count()
-                                                         : out->latency_s;
+    out->queue_s = r->admitted ? std::chrono::duration<double>(r->t_admit - r->t_submit).count() : out->latency_s;
 }
 
 }  // namespace
@@ -549,7 +619,7 @@ extern "C" qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, 
     }
     if (!s->cfg.mem_budget_bytes) s->cfg.mem_budget_bytes = (size_t)(0.6 * (double)free_b);
     if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 3;
-    if (s->cfg.max_lsm_streams <= 0) s->cfg.max_lsm_streams = 4;
+    if (s->cfg.max_lsm_streams <= 0 || s->cfg.max_lsm_streams > PUMP_MAX_REQ) s->cfg.max_lsm_streams = PUMP_MAX_REQ;
     if (s->cfg.item_path_steps <= 0) s->cfg.item_path_steps = 131072;
     if (s->cfg.blocks_per_cu <= 0) s->cfg.blocks_per_cu = 8;
     int n_cu = 0;
@@ -562,19 +632,16 @@ extern "C" qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, 
     s->slots.resize(s->cfg.max_inflight);
     for (Slot &sl : s->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.ev_fwd, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming) != hipSuccess) {
             qk_server_destroy(s);
             return fail(ctx, QK_EDEVICE, "stream creation failed");
         }
     }
-    s->astreams.resize(s->cfg.max_lsm_streams);
-    for (AStream &as : s->astreams) {
-        if (hipStreamCreateWithFlags(&as.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipHostMalloc((void **)&as.hbuf, 512, hipHostMallocDefault) != hipSuccess) {
-            qk_server_destroy(s);
-            return fail(ctx, QK_EDEVICE, "stream creation failed");
-        }
+    s->pslot_used.assign(PUMP_MAX_REQ, false);
+    if (hipStreamCreateWithFlags(&s->pump_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&s->pump_hbuf, PUMP_MAX_REQ * PUMP_SLOT_BYTES, hipHostMallocDefault) != hipSuccess) {
+        qk_server_destroy(s);
+        return fail(ctx, QK_EDEVICE, "pump stream creation failed");
     }
     s->th = std::thread(scheduler, s);
     *out = s;
@@ -592,22 +659,22 @@ extern "C" qk_status qk_server_destroy(qk_server *s) {
     (void)hipSetDevice(s->ctx->device);
     for (Slot &sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.ev_fwd) (void)hipEventDestroy(sl.ev_fwd);
         if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.hbuf) (void)hipHostFree(sl.hbuf);
         if (sl.hstats) (void)hipHostFree(sl.hstats);
         sl.dbuf.release();
     }
-    for (AStream &as : s->astreams) {
-        if (as.stream) {
-            (void)hipStreamSynchronize(as.stream);
-            (void)hipStreamDestroy(as.stream);
-        }
-        if (as.hbuf) (void)hipHostFree(as.hbuf);
+    if (s->pump_stream) {
+        (void)hipStreamSynchronize(s->pump_stream);
+        (void)hipStreamDestroy(s->pump_stream);
     }
+    for (hipEvent_t e : s->pump_steps) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    if (s->pump_hbuf) (void)hipHostFree(s->pump_hbuf);
     for (auto &kv : s->reqs) {
         if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
+        if (kv.second->ev_fwd) (void)hipEventDestroy(kv.second->ev_fwd);
         delete kv.second;
     }
     if (s->arena.base) (void)hipFree(s->arena.base);
@@ -635,9 +702,12 @@ extern "C" qk_status qk_server_submit(qk_server *s, const qk_contract *c, const 
     r->american = american;
     r->n_tiles = (c->n_paths + TILE - 1) / TILE;
     r->n_chunks = (r->n_tiles + MERGE_CHUNK - 1) / MERGE_CHUNK;
-    r->lay = layout_of(*c, american, r->n_tiles, r->n_chunks);
+    if (american) r->nb = pump_blocks(c->n_paths, &r->chunk);
+    r->lay = layout_of(*c, american, r->n_tiles, r->n_chunks, r->nb);
     r->t_submit = clk::now();
-    if (american && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) {
+    if (american && (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess ||
+                     hipEventCreateWithFlags(&r->ev_fwd, hipEventDisableTiming) != hipSuccess)) {
+        if (r->ev) (void)hipEventDestroy(r->ev);
         delete r;
         return fail(ctx, QK_EDEVICE, "hipEventCreate");
     }
@@ -660,6 +730,7 @@ static qk_status take_result(qk_server *s, uint64_t ticket, qk_result *out, bool
     }
     fill_result(r, out);
     if (r->ev) (void)hipEventDestroy(r->ev);
+    if (r->ev_fwd) (void)hipEventDestroy(r->ev_fwd);
     s->reqs.erase(it);
     delete r;
     return QK_OK;
