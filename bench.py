@@ -31,6 +31,34 @@ FLOPS_PER_PS = 49
 TRANS_PER_PS = 17
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes per launch of the FP32 path kernel on this workload, from the
+# FETCH_SIZE and WRITE_SIZE counter passes, each scaled by the same counter's
+# reading for a device copy of known size.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_paths_qe_fp32_4Mx365_hbm_traffic.txt")
+
+
+# VALU issue utilisation of the path kernels from the SQ counter passes.
+VALU_FILE = os.path.join(ROOT, "profiles", "k_paths_qe_4Mx365_valu_counters.md")
+# Semi-analytic Heston price of the European call of the Euler leg (S0=K=1,
+# T=1, r=0.03, v0=0.04, kappa=2, theta=0.04, sigma=0.5, rho=-0.7; Albrecher
+# characteristic function, one 64-point Gauss-Legendre panel on [0, 100]),
+# times the price scale 100.
+HESTON_CF_EUROPEAN = 8.929410423224049
+
+
+def _profile_value(path, key, conv):
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return conv(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def measured_traffic():
+    return _profile_value(TRAFFIC_FILE, "traffic_bytes_per_launch", int)
 
 
 def workload(precision, n_paths=N_PATHS):
@@ -87,6 +115,80 @@ def cpu_baseline(target_s=15.0):
             "sample": "%d paths x %d steps of the same QE workload (3 payoffs), FP64, %.1f s" % (n2, N_STEPS, el)}
 
 
+def other_configs(engine):
+    """The other single-GPU workloads at their full sizes: Euler European FP64
+    against the Heston characteristic-function price; Bates QE American put by
+    Longstaff-Schwartz (HBM path store, per-date Gram reduction and solve); the
+    Greek stencil as lanes of one launch.  Each is timed once after a warm-up."""
+    import quantkos_amd as qk
+    out = {}
+    # Heston European call, Euler full truncation, 1,000,000 paths x 252 steps.
+    progress("Euler European 1M x 252 (FP64, FP32)")
+    eu = {}
+    for prec, name in ((qk.FP64, "fp64"), (qk.FP32, "fp32")):
+        c = qk.contract(model=qk.HESTON, scheme=qk.EULER, precision=prec, n_steps=252, n_paths=1_000_000,
+                        seed=42, S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5,
+                        rho=-0.7, price_scale=100.0)
+        p = qk.payoff(qk.EUROPEAN, qk.CALL, 1.0)
+        engine.price(c, p)
+        t0 = time.perf_counter()
+        st = engine.price(c, p)
+        wall = time.perf_counter() - t0
+        tm = engine.timing()
+        eu[name] = {"price": st.price, "se": st.se, "wall_ms": wall * 1e3, "kernel_ms": tm.path_kernel_ms,
+                    "path_steps_per_s": 252e6 / wall,
+                    "err_in_se_vs_heston_cf": (st.price - HESTON_CF_EUROPEAN) / st.se}
+    eu["heston_cf"] = HESTON_CF_EUROPEAN
+    out["heston_euler_european_1Mx252"] = eu
+    # Bates QE American put K=110, 500,000 paths, 50 exercise dates.
+    progress("Bates QE American put 500k x 50 (FP32, FP64)")
+    am = {}
+    for prec, name, rs in ((qk.FP32, "fp32", 4), (qk.FP64, "fp64", 8)):
+        c = qk.contract(model=qk.BATES, scheme=qk.QE, precision=prec, n_steps=50, n_paths=500_000, seed=42,
+                        S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7,
+                        lambda_j=0.1, mu_j=-0.05, sigma_j=0.10, price_scale=100.0)
+        engine.price_american(c, qk.PUT, 1.1)
+        t0 = time.perf_counter()
+        st = engine.price_american(c, qk.PUT, 1.1)
+        wall = time.perf_counter() - t0
+        tm = engine.timing()
+        n, N = c.n_paths, c.n_steps
+        # backward pass bytes: per path and date, read S(t) and S(t-1) and
+        # read + write the cash flow
+        bw_bytes = n * N * 4 * rs
+        am[name] = {"price": st.price, "se": st.se, "wall_ms": wall * 1e3, "forward_ms": tm.path_kernel_ms,
+                    "backward_ms": tm.lsm_backward_ms, "merge_ms": tm.reduce_ms,
+                    "store_bytes": n * N * rs,
+                    "forward_store_gb_s": n * N * rs / (tm.path_kernel_ms * 1e-3) / 1e9,
+                    "backward_bytes": bw_bytes,
+                    "backward_gb_s": bw_bytes / (tm.lsm_backward_ms * 1e-3) / 1e9,
+                    "backward_us_per_date": tm.lsm_backward_ms * 1e3 / N}
+    am["fp32_minus_fp64_in_se"] = (am["fp32"]["price"] - am["fp64"]["price"]) / am["fp64"]["se"]
+    out["bates_qe_american_put_500kx50"] = am
+    # Greeks: Heston implicit Milstein, European call K=100 and cash-or-nothing
+    # call K=105 from one path set, 2,000,000 paths x 252 steps, S0 +-1 %,
+    # vol +-1e-3, r +-1bp, T -+1/365 (central), all bumps as lanes of one launch.
+    progress("Greeks 2M x 252, 13 lanes x 2 contracts (FP32)")
+    c = qk.contract(model=qk.HESTON, scheme=qk.MILSTEIN, precision=qk.FP32, n_steps=252, n_paths=2_000_000,
+                    seed=42, S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7,
+                    price_scale=100.0)
+    pays = [qk.payoff(qk.EUROPEAN, qk.CALL, 1.0), qk.payoff(qk.BINARY_CASH, qk.CALL, 1.05)]
+    b = qk.bumps(rel_S=0.01, abs_vol=1e-3, abs_r=1e-4, abs_T=1.0 / 365.0, theta_central=1)
+    engine.greeks(c, pays, b)
+    t0 = time.perf_counter()
+    g = engine.greeks(c, pays, b)
+    wall = time.perf_counter() - t0
+    tm = engine.timing()
+    lanes = g[0].n_lanes
+    names = ("delta", "gamma", "vega", "vomma", "vanna", "rho", "theta", "base_price", "base_se")
+    out["heston_milstein_greeks_2Mx252"] = {
+        "lanes": lanes, "contracts": len(pays), "wall_ms": wall * 1e3, "kernel_ms": tm.path_kernel_ms,
+        "path_steps_per_s": lanes * c.n_paths * c.n_steps / wall,
+        "european_call": {k: getattr(g[0], k) for k in names},
+        "binary_cash_call": {k: getattr(g[1], k) for k in names}}
+    return out
+
+
 def progress(msg):
     print("[bench %s] %s" % (time.strftime("%H:%M:%S"), msg), file=sys.stderr, flush=True)
 
@@ -131,7 +233,7 @@ def serving(engine, n_requests=4096, fp64_fraction=0.25):
              % (wall, out["poisson"]["p50_ms"], out["poisson"]["p99_ms"]))
     step = max(1, int(round(1.0 / fp64_fraction)))
     idx = list(range(0, len(cs), step))
-    err = []
+    err, err_am = [], []
     ps64 = 0
     t0 = time.perf_counter()
     for done, i in enumerate(idx):
@@ -144,13 +246,22 @@ def serving(engine, n_requests=4096, fp64_fraction=0.25):
         if done % 128 == 0:
             progress("serving: plain FP64 %d/%d" % (done, len(idx)))
         if s.se > 0:
-            err.append(abs(burst[i].stats.price - s.price) / s.se)
+            (err_am if p.type == qk.AMERICAN else err).append(abs(burst[i].stats.price - s.price) / s.se)
     wall = time.perf_counter() - t0
     out["fp64_plain"] = {"requests": len(idx), "wall_s": wall, "path_steps_per_s": ps64 / wall}
     out["fp32_batched_over_fp64_plain"] = out["burst"]["path_steps_per_s"] / out["fp64_plain"]["path_steps_per_s"]
-    err.sort()
-    out["fp32_err_in_se_vs_fp64"] = {"max": err[-1], "p99": err[int(0.99 * (len(err) - 1))],
-                                     "within_1se": sum(e <= 1.0 for e in err) / len(err)}
+    def summary(e):
+        e = sorted(e)
+        if not e:
+            return None
+        return {"n": len(e), "max": e[-1], "p99": e[int(0.99 * (len(e) - 1))],
+                "within_1se": sum(x <= 1.0 for x in e) / len(e)}
+    # FP32 batched vs FP64 plain, same random numbers.  For American requests
+    # the two precisions may settle on different exercise policies, so their
+    # difference also holds the policy's own sampling noise, which the
+    # reported SE (cash flows under a fixed policy) does not measure.
+    out["fp32_err_in_se_vs_fp64"] = summary(err)
+    out["fp32_err_in_se_vs_fp64_american"] = summary(err_am)
     return out
 
 
@@ -179,6 +290,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=4096)
     ap.add_argument("--serving-fp64-fraction", type=float, default=0.25)
     args = ap.parse_args()
@@ -192,6 +304,7 @@ def main():
     progress("config 1 FP32: %.2f ms per pricing; FP64" % (dt32 * 1e3 / args.steps))
     s64 = args.fp64_steps or max(1, args.steps // 2)
     r64, dt64, k64 = run(eng, qk.FP64, s64, 1)
+    others = None if args.no_other_configs else other_configs(eng)
     serve = None if args.no_serving else serving(eng, args.serving_requests, args.serving_fp64_fraction)
     eng.close()
     ps = N_PATHS * N_STEPS
@@ -210,16 +323,22 @@ def main():
                                "call K=100 B=130 daily, floating-strike lookback put",
                    "paths": N_PATHS, "time_steps": N_STEPS, "parallelism": "1 GPU"},
         "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": ach / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "k_paths<float,Heston,QE,pathdep>", "kernel_ms": kavg32,
+                     "valu_issue_util": _profile_value(VALU_FILE, "valu_issue_util_fp32", float),
+                     "valu_instructions_per_path_step": _profile_value(
+                         VALU_FILE, "valu_instructions_per_path_step_fp32", float),
                      "flops_per_path_step": FLOPS_PER_PS, "transcendentals_per_path_step": TRANS_PER_PS},
         "fp64": {"value": v64, "ms_per_step": dt64 * 1e3 / s64, "kernel_ms": kavg64, "steps": s64,
                  "achieved_tflops": FLOPS_PER_PS * ps / (kavg64 * 1e-3) / 1e12,
-                 "peak": PEAK_FP64_TFLOPS},
+                 "peak": PEAK_FP64_TFLOPS,
+                 "valu_issue_util": _profile_value(VALU_FILE, "valu_issue_util_fp64", float)},
         "fp32_over_fp64": v32 / v64,
         "prices": {"fp32": [round(a.price, 6) for a in r32], "fp64": [round(b.price, 6) for b in r64],
                    "se_fp64": [round(b.se, 6) for b in r64], "fp32_err_in_se": [round(e, 4) for e in err_se]},
     }
+    if others is not None:
+        line["other_configs"] = others
     if serve is not None:
         line["serving"] = serve
     if not args.no_cpu_baseline:

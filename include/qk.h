@@ -104,6 +104,20 @@ qk_status qk_last_timing(const qk_ctx *ctx, qk_timing *out);
 /* Bytes of device memory a call with this contract / payload would need. */
 size_t qk_footprint_bytes(const qk_contract *c, int32_t n_lanes, int32_t n_payoffs, int32_t american);
 
+/* ---- launch schedule (the autotuner's knob for the lane kernels) ----
+ * How many lanes of a qk_price_lanes / qk_price_greeks launch one thread
+ * carries, sharing each step's random numbers: 1, 2 or 4.  Entries are keyed
+ * by (precision, scheme, n_lanes); a call without a matching entry uses 4 when
+ * n_lanes >= 4, else 1.  The schedule changes speed only: prices agree to
+ * rounding whatever it is.  Replaces the launch-policy choice the reference
+ * leaves to Kokkos' RangePolicy (MSolver.hpp:195-232); quantkos_amd/autotune.py
+ * measures the candidates and writes the table. */
+typedef struct { int32_t precision, scheme, n_lanes, lanes_per_thread; } qk_schedule_entry;
+/* Replaces the ctx's table with entries[0..n) (n = 0 clears it; at most 256). */
+qk_status qk_set_schedule(qk_ctx *ctx, const qk_schedule_entry *entries, int32_t n);
+/* The lanes per thread a call with this key would use. */
+int32_t qk_lanes_per_thread(const qk_ctx *ctx, int32_t precision, int32_t scheme, int32_t n_lanes);
+
 /* ---- pricing ---- */
 /* Plain single-contract launch.  Replaces MSolver<M,S,T,R>::execute_batch
  * (MSolver.hpp:195-232) + OptionPricer::accumulate_batch_payoffs /

@@ -74,6 +74,11 @@ class ServerConfig(C.Structure):
                 ("blocks_per_cu", C.c_int32)]
 
 
+class ScheduleEntry(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("scheme", C.c_int32), ("n_lanes", C.c_int32),
+                ("lanes_per_thread", C.c_int32)]
+
+
 class Result(C.Structure):
     _fields_ = [("stats", Stats), ("latency_s", C.c_double), ("queue_s", C.c_double),
                 ("status", C.c_int)]
@@ -108,6 +113,8 @@ def load():
         "qk_synchronize": ([vp], st),
         "qk_last_timing": ([vp, P(Timing)], st),
         "qk_footprint_bytes": ([P(Contract), C.c_int32, C.c_int32, C.c_int32], C.c_size_t),
+        "qk_set_schedule": ([vp, P(ScheduleEntry), C.c_int32], st),
+        "qk_lanes_per_thread": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int32),
         "qk_price": ([vp, P(Contract), P(Payoff), P(Stats)], st),
         "qk_price_payoffs": ([vp, P(Contract), P(Payoff), C.c_int32, P(Stats)], st),
         "qk_price_lanes": ([vp, P(Contract), C.c_int32, P(Payoff), C.c_int32, P(Stats)], st),

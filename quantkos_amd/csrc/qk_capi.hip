@@ -296,7 +296,7 @@ template <typename Real>
 static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay) {
     const qk_contract &c0 = lanes[0];
     const bool pathdep = std::any_of(pays, pays + n_pay, [](const qk_payoff &p) { return path_dependent(p.type); });
-    const int L = n_lanes >= 4 ? 4 : 1;
+    const int L = qk_lanes_per_thread(ctx, c0.precision, c0.scheme, n_lanes);
     const int n_padded = (n_lanes + L - 1) / L * L;
     std::vector<LaneC<Real>> hl(n_padded);
     std::vector<double> hdisc(n_lanes), hscale(n_pay);
@@ -388,6 +388,26 @@ static qk_status validate_lanes(qk_ctx *ctx, const qk_contract *lanes, int32_t n
 }
 
 }  // namespace qk
+
+extern "C" qk_status qk_set_schedule(qk_ctx *ctx, const qk_schedule_entry *entries, int32_t n) {
+    if (!ctx) return QK_EINVAL;
+    if (n < 0 || n > 256 || (n > 0 && !entries)) return fail(ctx, QK_EINVAL, "schedule: 0 <= n <= 256 entries");
+    for (int32_t i = 0; i < n; ++i) {
+        const qk_schedule_entry &e = entries[i];
+        if (e.lanes_per_thread != 1 && e.lanes_per_thread != 2 && e.lanes_per_thread != 4)
+            return fail(ctx, QK_EINVAL, "schedule: lanes_per_thread must be 1, 2 or 4");
+        if (e.n_lanes < 1 || e.n_lanes > MAX_LANES) return fail(ctx, QK_EINVAL, "schedule: n_lanes must be in [1, 64]");
+    }
+    ctx->schedule.assign(entries, entries + n);
+    return QK_OK;
+}
+
+extern "C" int32_t qk_lanes_per_thread(const qk_ctx *ctx, int32_t precision, int32_t scheme, int32_t n_lanes) {
+    if (ctx)
+        for (const qk_schedule_entry &e : ctx->schedule)
+            if (e.precision == precision && e.scheme == scheme && e.n_lanes == n_lanes) return e.lanes_per_thread;
+    return n_lanes >= 4 ? 4 : 1;
+}
 
 extern "C" qk_status qk_price_lanes(qk_ctx *ctx, const qk_contract *lanes, int32_t n_lanes, const qk_payoff *pays,
                                     int32_t n_pay, qk_stats *out) {

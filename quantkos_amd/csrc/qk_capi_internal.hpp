@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "qk_types.hpp"
 
@@ -38,5 +39,6 @@ struct qk_ctx {
     std::string err;
     qk_timing timing{};
     qk::DevBuf lanes, pays, psum, pm2, pitm, disc, scale, stats, greeks, store, cf, coeffs, bpart, counter, sums, mscratch;
+    std::vector<qk_schedule_entry> schedule;   // lanes per thread by (precision, scheme, n_lanes)
     ~qk_ctx();
 };

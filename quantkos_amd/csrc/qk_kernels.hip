@@ -529,6 +529,9 @@ static void launch_ms(const PathArgs<Real> &a, bool pathdep, bool store, int L, 
     } else if (L == 4) {
         if (pathdep) hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, false, 4>), grid, dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, false, false, 4>), grid, dim3(BLOCK), 0, s, a);
+    } else if (L == 2) {
+        if (pathdep) hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, false, 2>), grid, dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, false, false, 2>), grid, dim3(BLOCK), 0, s, a);
     } else {
         if (pathdep) hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, false, 1>), grid, dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, false, false, 1>), grid, dim3(BLOCK), 0, s, a);

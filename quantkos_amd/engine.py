@@ -9,7 +9,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Bumps, Contract, Greeks, Payoff, Result, ServerConfig, ServerStats, Stats, Timing
+from ._lib import (Bumps, Contract, Greeks, Payoff, Result, ScheduleEntry, ServerConfig, ServerStats, Stats,
+                   Timing)
 
 
 class QKError(RuntimeError):
@@ -70,15 +71,36 @@ def run_config(path: str, device: int = 0, quiet: bool = True):
 
 
 class Engine:
-    """One qk_ctx: a device, a memory budget and a stream."""
+    """One qk_ctx: a device, a memory budget and a stream.
 
-    def __init__(self, device: int = 0, mem_budget_bytes: int = 0):
+    `tuning`: a table written by quantkos_amd.autotune (a dict or a JSON file
+    path); "default" loads the package's table for this device architecture
+    when there is one, None applies nothing.  The table sets the lane kernels'
+    lanes per thread and the request server's default schedule."""
+
+    def __init__(self, device: int = 0, mem_budget_bytes: int = 0, tuning="default"):
         self.L = _lib.load()
         h = C.c_void_p()
         st = self.L.qk_ctx_create(device, mem_budget_bytes, C.byref(h))
         if st != 0:
             raise QKError(st, "qk_ctx_create(device=%d)" % device)
         self.h = h
+        self.tuning = None
+        if tuning is not None:
+            from . import autotune
+            table = autotune.load() if tuning == "default" else (
+                tuning if isinstance(tuning, dict) else autotune.load(tuning))
+            if table is not None:
+                autotune.apply(self, table)
+
+    def set_schedule(self, entries):
+        """entries: iterable of (precision, scheme, n_lanes, lanes_per_thread)."""
+        entries = list(entries)
+        arr = (ScheduleEntry * max(1, len(entries)))(*[ScheduleEntry(*e) for e in entries])
+        self._check(self.L.qk_set_schedule(self.h, arr, len(entries)))
+
+    def lanes_per_thread(self, precision: int, scheme: int, n_lanes: int) -> int:
+        return self.L.qk_lanes_per_thread(self.h, precision, scheme, n_lanes)
 
     def close(self):
         if self.h:
@@ -150,6 +172,10 @@ class Server:
                  max_lsm_streams: int = 0, item_path_steps: int = 0, blocks_per_cu: int = 0):
         self.engine = engine
         self.L = engine.L
+        tuned = (engine.tuning or {}).get("server") or {}
+        max_inflight = max_inflight or int(tuned.get("max_inflight", 0))
+        item_path_steps = item_path_steps or int(tuned.get("item_path_steps", 0))
+        blocks_per_cu = blocks_per_cu or int(tuned.get("blocks_per_cu", 0))
         cfg = ServerConfig(mem_budget_bytes, max_inflight, max_lsm_streams, item_path_steps, blocks_per_cu)
         h = C.c_void_p()
         engine._check(self.L.qk_server_create(engine.h, C.byref(cfg), C.byref(h)))

@@ -324,3 +324,22 @@ def test_greek_parity_is_exact_under_common_random_numbers(engine):
         lhs = getattr(gc, k) - getattr(gp, k)
         rhs = getattr(gf, k) + extra
         assert math.isclose(lhs, rhs, rel_tol=1e-7, abs_tol=1e-7 * (1 + abs(getattr(gc, k)))), (k, lhs, rhs)
+
+
+def test_config0_contract(engine, oracle):
+    """The Euler European of the Heston characteristic-function check (S0=K=100, T=1, r=0.03,
+    v0=0.04, kappa=2, theta=0.04, sigma=0.5, rho=-0.7, 252 steps, FP64): Euler with the
+    reference's absorbed variance equals the FP64 oracle on the same random numbers, and QE on the
+    same contract lands within 3 SE of the characteristic-function price (8.92941)."""
+    kw = dict(model=qk.HESTON, precision=qk.FP64, n_steps=252, seed=42, S0=1.0, v0=0.04, r=0.03, q=0.0,
+              T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7, price_scale=100.0)
+    p = qe.payoff(qk.EUROPEAN, qk.CALL, 1.0)
+    ce = qe.contract(scheme=qk.EULER, n_paths=200_000, **kw)
+    eu = engine.price(ce, p)
+    o = oracle.price(omodel(oracle, ce), ce.n_paths, [opay(oracle, p)], 100.0)[0]
+    assert math.isclose(eu.price, o.price, rel_tol=1e-9), (eu.price, o.price)
+    assert eu.itm == o.itm
+    cf = 100.0 * oracle.heston_call(1.0, 1.0, 1.0, 0.03, 0.0, 0.04, 2.0, 0.04, 0.5, -0.7)
+    assert abs(cf - 8.92941) < 1e-4
+    q = engine.price(qe.contract(scheme=qk.QE, n_paths=1_000_000, **kw), p)
+    assert abs(q.price - cf) <= 3.0 * q.se, (q.price, cf, q.se)
