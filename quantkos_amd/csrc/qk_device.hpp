@@ -18,14 +18,20 @@ namespace qk {
 // ---------------------------------------------------------------------------
 struct Words { uint32_t w0, w1, w2, w3; };
 
+// Three-input XOR in one gfx950 v_bitop3_b32 (truth table 0x96); the compiler
+// does not form it from two XORs by itself.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ Words philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                         uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
@@ -48,6 +54,20 @@ template <> struct Math<float> {
     __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static __forceinline__ float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
     __device__ static __forceinline__ float fmin(float a, float b) { return __builtin_fminf(a, b); }
+    // Running max / min of non-negative values (spot prices) on their bit
+    // patterns: the same result as fmax / fmin, without the NaN-quieting
+    // operand fix-ups the compiler adds to a loop-carried fmax.
+    __device__ static __forceinline__ float track_max(float a, float b) {
+        return __uint_as_float(__builtin_elementwise_max(__float_as_uint(a), __float_as_uint(b)));
+    }
+    __device__ static __forceinline__ float track_min(float a, float b) {
+        return __uint_as_float(__builtin_elementwise_min(__float_as_uint(a), __float_as_uint(b)));
+    }
+    // -2 ln u as log2(u) * (-2 ln 2): scaling by -2 commutes with rounding, so
+    // this equals -2 * log(u) exactly, one multiply shorter.
+    __device__ static __forceinline__ float neg2log(float u) {
+        return __builtin_amdgcn_logf(u) * (-2.0f * 0.6931471805599453f);
+    }
     // sin/cos of 2*pi*u for u in (0, 1).
     __device__ static __forceinline__ void sincos2pi(float u, float &s, float &c) {
         s = __builtin_amdgcn_sinf(u);
@@ -67,6 +87,9 @@ template <> struct Math<double> {
     __device__ static __forceinline__ double div(double a, double b) { return a / b; }
     __device__ static __forceinline__ double fmax(double a, double b) { return ::fmax(a, b); }
     __device__ static __forceinline__ double fmin(double a, double b) { return ::fmin(a, b); }
+    __device__ static __forceinline__ double track_max(double a, double b) { return ::fmax(a, b); }
+    __device__ static __forceinline__ double track_min(double a, double b) { return ::fmin(a, b); }
+    __device__ static __forceinline__ double neg2log(double u) { return -2.0 * ::log(u); }
     __device__ static __forceinline__ void sincos2pi(double u, double &s, double &c) {
         ::sincospi(2.0 * u, &s, &c);
     }
@@ -218,7 +241,7 @@ __device__ __noinline__ Real bates_jump_slow(const LaneC<Real> &c, Real u3, uint
     const Real e1 = bates_extra<Real>(p0, p1, t, j + 1, k0, k1);
     Real s, co;
     M::sincos2pi(e1, s, co);
-    const Real z = M::sqrt(Real(-2) * M::log(e0)) * co;
+    const Real z = M::sqrt(M::neg2log(e0)) * co;
     const Real n = Real(N);
     return n * c.mu_j + M::sqrt(n) * c.sigma_j * z;
 }

@@ -61,7 +61,7 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
         const Real u0 = M::uniform(w.w0), u1 = M::uniform(w.w1);
         Real sn, cs;
         M::sincos2pi(u1, sn, cs);
-        const Real R = M::sqrt(Real(-2) * M::log(u0));
+        const Real R = M::sqrt(M::neg2log(u0));
         const Real z1 = R * cs, z2 = R * sn;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
@@ -73,8 +73,8 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
             if constexpr (PATHDEP) {
                 S[l] *= M::exp(inc);
                 sm[l] += (double)S[l];
-                mx[l] = M::fmax(mx[l], S[l]);
-                mn[l] = M::fmin(mn[l], S[l]);
+                mx[l] = M::track_max(mx[l], S[l]);
+                mn[l] = M::track_min(mn[l], S[l]);
             } else {
                 S[l] += inc;
             }
