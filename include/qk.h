@@ -151,6 +151,23 @@ qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk_payoff *pa
 qk_status qk_price_american(qk_ctx *ctx, const qk_contract *c, int32_t right, double K,
                             qk_stats *out, double *coeffs_out);
 
+/* Discounted payoff distribution (OptionPricer::analyze_risk_neutral_discounted_
+ * payoff_distribution, OptionPricer.hpp:340-360, enabled by the YAML key
+ * MC.analyze_risk_neutral_payoff_distribution): the per-path payoffs sorted
+ * ascending, taken at 0-based ranks floor(n*q) for q = 0.01, 0.05, 0.10, ...,
+ * 0.90, 0.95, 0.99 (FP64 product, truncated), plus the minimum and maximum,
+ * each times e^{-rT} (none for American) and the price scale.  The values are
+ * exact order statistics, selected on the device. */
+typedef struct {
+    double p0, p1, p5, p10, p20, p30, p40, median, p60, p70, p80, p90, p95, p99, p100;
+    double select_s;   /* wall time of the selection (the reference's sorting time) */
+} qk_distribution;
+
+/* qk_price (or, for payoff->type QK_AMERICAN, qk_price_american) plus the
+ * payoff distribution of the same paths. */
+qk_status qk_price_distribution(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoff, qk_stats *out,
+                                qk_distribution *dist);
+
 /* Simulated spot paths [path0, path0 + c->n_paths) of the path set, copied to
  * host memory laid out [step][path] (S after step t+1 at row t), as the
  * reference's StorePaths kernel writes them (MSolver.hpp:117-119).  Path p
@@ -219,6 +236,7 @@ typedef struct {
     qk_stats stats;            /* price, se, prob_itm, n, itm                       */
     qk_greeks greeks;          /* zero unless Greeks were requested                 */
     double t_total_s, t_pricing_s, t_greeks_s;   /* the reference's MCTimer lines  */
+    qk_distribution dist;      /* MCOpPrices p0 ... p100 (-999 unless MC.analyze_risk_neutral_payoff_distribution) */
 } qk_run_result;
 
 /* MCDispatcher(Parser::parse(file)).launch_montecarlo() (app/main.cpp:78-93,

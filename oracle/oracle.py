@@ -79,7 +79,7 @@ def lib():
         L.qko_payoff_eval.argtypes = [P(Payoff), d, d, d, d, C.c_int32]
         L.qko_payoff_eval.restype = d
         L.qko_price.argtypes = [P(Model), C.c_int64, P(Payoff), C.c_int32, d, P(Stats)]
-        L.qko_price_american.argtypes = [P(Model), C.c_int64, C.c_int32, d, d, P(Stats), P(d)]
+        L.qko_price_american.argtypes = [P(Model), C.c_int64, C.c_int32, d, d, P(Stats), P(d), P(d)]
         L.qko_greeks_eval.argtypes = [P(Model), C.c_int64, P(Payoff), P(Bumps), d, P(Greeks)]
         L.qko_chan_merge.argtypes = [P(d), P(d), P(d), d, d, d]
         L.qko_lsm_solve.argtypes = [P(d), P(d)]
@@ -154,14 +154,32 @@ def price(m: Model, n_paths: int, payoffs, price_scale: float = 1.0):
 
 
 def price_american(m: Model, n_paths: int, right: int, K: float, price_scale: float = 1.0,
-                   want_coeffs=False):
+                   want_coeffs=False, want_cf=False):
+    """Stats; with want_coeffs also the coefficients [n_steps][4]; with want_cf
+    also the final discounted cash flows per path (after the day-0 rule)."""
     out = Stats()
     co = np.zeros((m.n_steps, 4)) if want_coeffs else None
+    cf = np.zeros(n_paths) if want_cf else None
     rc = lib().qko_price_american(C.byref(m), n_paths, right, K, price_scale, C.byref(out),
-                                  _dptr(co))
+                                  _dptr(co), _dptr(cf))
     if rc != 0:
         raise ValueError("qko_price_american failed")
-    return (out, co) if want_coeffs else out
+    res = (out,) + ((co,) if want_coeffs else ()) + ((cf,) if want_cf else ())
+    return res if len(res) > 1 else out
+
+
+# Rank fractions of the payoff distribution (OptionPricer.hpp:345-357).
+PERCENTILE_Q = (0.01, 0.05, 0.10, 0.20, 0.30, 0.40, 0.50, 0.60, 0.70, 0.80, 0.90, 0.95, 0.99)
+
+
+def percentiles(values, factor=1.0):
+    """analyze_risk_neutral_discounted_payoff_distribution (OptionPricer.hpp:340-360):
+    sort ascending; p0 = first, p_q = element floor(n*q) (FP64 product,
+    truncated), p100 = last; each times `factor` (discount x scale)."""
+    v = np.sort(np.asarray(values, dtype=np.float64))
+    n = len(v)
+    ranks = [0] + [int(float(n) * q) for q in PERCENTILE_Q] + [n - 1]
+    return [float(v[r]) * factor for r in ranks]
 
 
 def greeks(m: Model, n_paths: int, p: Payoff, b: Bumps, price_scale: float = 1.0) -> Greeks:

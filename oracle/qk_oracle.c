@@ -441,7 +441,7 @@ int qko_lsm_solve(const double sums[21], double C[4]) {
 }
 
 int qko_price_american(const qko_model *m, int64_t n_paths, int32_t right, double K,
-                       double price_scale, qko_stats *out, double *coeffs_out) {
+                       double price_scale, qko_stats *out, double *coeffs_out, double *cf_out) {
     if (!m || !out || n_paths < 1 || m->n_steps < 2) return -1;
     const int32_t N = m->n_steps;
     double *S = malloc(sizeof(double) * n_paths * (size_t)N);
@@ -496,6 +496,7 @@ int qko_price_american(const qko_model *m, int64_t n_paths, int32_t right, doubl
     int64_t itm;
     moments(cf, n_paths, m->fp32_semantics ? 1e-6 : 1e-12, &mean, &m2, &itm);
     finish(out, n_paths, mean, m2, itm, 1.0, price_scale);   /* no e^{-rT}: OptionPricer.hpp:211-216 */
+    if (cf_out) memcpy(cf_out, cf, sizeof(double) * (size_t)n_paths);
     free(S);
     free(cf);
     return 0;

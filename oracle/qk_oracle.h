@@ -95,7 +95,7 @@ double qko_payoff_eval(const qko_payoff *p, double S_T, double S_sum, double S_m
 int qko_price(const qko_model *m, int64_t n_paths, const qko_payoff *pay, int32_t n_pay,
               double price_scale, qko_stats *out);
 int qko_price_american(const qko_model *m, int64_t n_paths, int32_t right, double K,
-                       double price_scale, qko_stats *out, double *coeffs_out);
+                       double price_scale, qko_stats *out, double *coeffs_out, double *cf_out);
 int qko_greeks_eval(const qko_model *m, int64_t n_paths, const qko_payoff *pay,
                     const qko_bumps *b, double price_scale, qko_greeks *out);
 

@@ -58,6 +58,11 @@ class Timing(C.Structure):
                 ("path_steps", C.c_int64)]
 
 
+class Distribution(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("p0", "p1", "p5", "p10", "p20", "p30", "p40", "median", "p60", "p70",
+                                          "p80", "p90", "p95", "p99", "p100", "select_s")]
+
+
 class RunSpec(C.Structure):
     _fields_ = [("contract", Contract), ("payoff", Payoff), ("bumps", Bumps),
                 ("compute_greeks", C.c_int32), ("write_paths", C.c_int32)]
@@ -65,7 +70,7 @@ class RunSpec(C.Structure):
 
 class RunResult(C.Structure):
     _fields_ = [("stats", Stats), ("greeks", Greeks), ("t_total_s", C.c_double),
-                ("t_pricing_s", C.c_double), ("t_greeks_s", C.c_double)]
+                ("t_pricing_s", C.c_double), ("t_greeks_s", C.c_double), ("dist", Distribution)]
 
 
 class ServerConfig(C.Structure):
@@ -121,6 +126,7 @@ def load():
         "qk_price_greeks": ([vp, P(Contract), P(Payoff), C.c_int32, P(Bumps), P(Greeks)], st),
         "qk_price_american": ([vp, P(Contract), C.c_int32, C.c_double, P(Stats), P(C.c_double)], st),
         "qk_simulate_paths": ([vp, P(Contract), C.c_int64, P(C.c_double)], st),
+        "qk_price_distribution": ([vp, P(Contract), P(Payoff), P(Stats), P(Distribution)], st),
         "qk_server_create": ([vp, P(ServerConfig), P(vp)], st),
         "qk_server_destroy": ([vp], st),
         "qk_server_submit": ([vp, P(Contract), P(Payoff), P(C.c_uint64)], st),

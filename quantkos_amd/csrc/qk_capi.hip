@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -253,7 +254,7 @@ extern "C" qk_status qk_ctx_create(int device, size_t mem_budget_bytes, qk_ctx *
 qk_ctx::~qk_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (DevBuf *b : {&lanes, &pays, &psum, &pm2, &pitm, &disc, &scale, &stats, &greeks, &store, &cf, &coeffs,
-                      &bpart, &counter, &sums, &mscratch})
+                      &bpart, &counter, &sums, &mscratch, &payv, &hist})
         b->release();
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
@@ -293,7 +294,8 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 template <typename Real>
-static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay) {
+static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
+                           Real *pay_out = nullptr) {
     const qk_contract &c0 = lanes[0];
     const bool pathdep = std::any_of(pays, pays + n_pay, [](const qk_payoff &p) { return path_dependent(p.type); });
     const int L = qk_lanes_per_thread(ctx, c0.precision, c0.scheme, n_lanes);
@@ -341,6 +343,7 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     a.part_m2 = (double *)ctx->pm2.p;
     a.part_itm = (uint32_t *)ctx->pitm.p;
     a.store = nullptr;
+    a.pay_out = pay_out;
 
     QK_HIP(hipEventRecord(ctx->ev[0], s));
     QK_HIP(launch_paths<Real>(a, c0.model, c0.scheme, pathdep, false, L, s));
@@ -622,6 +625,134 @@ extern "C" qk_status qk_price_american(qk_ctx *ctx, const qk_contract *c, int32_
                                  : run_american<double>(ctx, *c, right, K, out, coeffs_out);
     if (st != QK_OK) return st;
     return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, true);
+}
+
+// ---------------------------------------------------------------------------
+// Payoff distribution (OptionPricer.hpp:340-360): exact order statistics of
+// the per-path values at the reference's ranks, selected on the device.
+// ---------------------------------------------------------------------------
+namespace qk {
+
+constexpr int N_QUANTILES = 15;
+// Rank fractions of p1 ... p99 in qk_distribution order (OptionPricer.hpp:345-357).
+constexpr double QUANTILE_Q[N_QUANTILES - 2] = {0.01, 0.05, 0.10, 0.20, 0.30, 0.40, 0.50,
+                                                 0.60, 0.70, 0.80, 0.90, 0.95, 0.99};
+
+template <typename Real>
+static Real key_to_value(uint64_t k) {
+    if constexpr (sizeof(Real) == 4) {
+        uint32_t b = (uint32_t)k;
+        b = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
+        float f;
+        std::memcpy(&f, &b, 4);
+        return f;
+    } else {
+        uint64_t b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+        double d;
+        std::memcpy(&d, &b, 8);
+        return d;
+    }
+}
+
+// Values of the given ranks (0-based, ascending order) of vals[0..n).
+template <typename Real>
+static qk_status select_ranks(qk_ctx *ctx, const Real *vals, int64_t n, const int64_t *ranks, int n_ranks,
+                              double *out) {
+    constexpr int BITS = 8 * (int)sizeof(Real);
+    qk_status st;
+    if ((st = ctx->hist.reserve(ctx, SELECT_MAX_GROUPS * 256 * 4)) != QK_OK) return st;
+    std::vector<uint64_t> prefix(n_ranks, 0);
+    std::vector<int64_t> rest(ranks, ranks + n_ranks);
+    std::vector<uint32_t> h(SELECT_MAX_GROUPS * 256);
+    int n_cu = 256;
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)n_cu * 8));
+    hipStream_t s = ctx->stream;
+    for (int shift = BITS - 8; shift >= 0; shift -= 8) {
+        SelectArgs a{};
+        std::vector<int> group(n_ranks);
+        for (int r = 0; r < n_ranks; ++r) {
+            int g = 0;
+            while (g < a.n_groups && a.prefix[g] != prefix[r]) ++g;
+            if (g == a.n_groups) a.prefix[a.n_groups++] = prefix[r];
+            group[r] = g;
+        }
+        QK_HIP(hipMemsetAsync(ctx->hist.p, 0, (size_t)a.n_groups * 256 * 4, s));
+        a.shift = shift;
+        QK_HIP(launch_select_hist<Real>(vals, n, a, (uint32_t *)ctx->hist.p, blocks, s));
+        QK_HIP(hipMemcpyAsync(h.data(), ctx->hist.p, (size_t)a.n_groups * 256 * 4, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < n_ranks; ++r) {
+            const uint32_t *hg = h.data() + group[r] * 256;
+            int d = 0;
+            while (d < 255 && rest[r] >= (int64_t)hg[d]) rest[r] -= hg[d++];
+            prefix[r] = (prefix[r] << 8) | (uint64_t)d;
+        }
+    }
+    for (int r = 0; r < n_ranks; ++r) out[r] = (double)key_to_value<Real>(prefix[r]);
+    return QK_OK;
+}
+
+template <typename Real>
+static qk_status run_distribution(qk_ctx *ctx, const qk_contract &c, const qk_payoff &p, qk_stats *out,
+                                  qk_distribution *dist) {
+    const int64_t n = c.n_paths;
+    const bool american = p.type == QK_AMERICAN;
+    qk_status st;
+    const Real *vals;
+    double factor;
+    if (american) {
+        if ((st = run_american<Real>(ctx, c, p.right, p.K, out, nullptr)) != QK_OK) return st;
+        if ((st = finish_timing(ctx, n * (int64_t)c.n_steps, true)) != QK_OK) return st;
+        vals = (const Real *)ctx->cf.p;
+        factor = c.price_scale;                               // no e^{-rT}: OptionPricer.hpp:211-216
+    } else {
+        if ((st = ctx->payv.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
+        if ((st = run_lanes<Real>(ctx, &c, 1, &p, 1, (Real *)ctx->payv.p)) != QK_OK) return st;
+        QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, ctx->stream));
+        QK_HIP(hipStreamSynchronize(ctx->stream));
+        if ((st = finish_timing(ctx, n * (int64_t)c.n_steps, false)) != QK_OK) return st;
+        vals = (const Real *)ctx->payv.p;
+        factor = std::exp(-c.r * c.T) * (p.type == QK_BINARY_CASH ? 1.0 : c.price_scale);
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    int64_t ranks[N_QUANTILES];
+    const double N = (double)n;
+    ranks[0] = 0;
+    for (int i = 0; i < N_QUANTILES - 2; ++i) ranks[1 + i] = (int64_t)(N * QUANTILE_Q[i]);
+    ranks[N_QUANTILES - 1] = n - 1;
+    double v[N_QUANTILES];
+    // Immediate exercise at t0 replaces every cash flow (LSMAmerican.hpp:409-418).
+    if (american && out->se == 0.0 && out->m2 == 0.0 && n > 1) {
+        for (double &x : v) x = out->mean;
+    } else if ((st = select_ranks<Real>(ctx, vals, n, ranks, N_QUANTILES, v)) != QK_OK) {
+        return st;
+    }
+    double *d = &dist->p0;
+    for (int i = 0; i < N_QUANTILES; ++i) d[i] = v[i] * factor;
+    dist->select_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return QK_OK;
+}
+
+}  // namespace qk
+
+extern "C" qk_status qk_price_distribution(qk_ctx *ctx, const qk_contract *c, const qk_payoff *p, qk_stats *out,
+                                           qk_distribution *dist) {
+    if (!ctx || !out || !dist || !p) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_contract(ctx, c);
+    if (st != QK_OK) return st;
+    const bool american = p->type == QK_AMERICAN;
+    if (american) {
+        if (p->right != QK_CALL && p->right != QK_PUT) return fail(ctx, QK_EINVAL, "payoff.right out of range");
+        if (!(p->K > 0.0)) return fail(ctx, QK_EINVAL, "payoff.K must be positive");
+    } else if ((st = validate_payoff(ctx, p)) != QK_OK) {
+        return st;
+    }
+    const size_t extra = american ? 0 : (size_t)c->n_paths * (c->precision == QK_FP32 ? 4 : 8);
+    if ((st = check_budget(ctx, footprint(*c, 1, 1, american) + extra)) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    return c->precision == QK_FP32 ? run_distribution<float>(ctx, *c, *p, out, dist)
+                                   : run_distribution<double>(ctx, *c, *p, out, dist);
 }
 
 extern "C" qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, int64_t path0, double *S_out) {

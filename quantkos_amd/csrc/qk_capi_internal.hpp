@@ -39,6 +39,7 @@ struct qk_ctx {
     std::string err;
     qk_timing timing{};
     qk::DevBuf lanes, pays, psum, pm2, pitm, disc, scale, stats, greeks, store, cf, coeffs, bpart, counter, sums, mscratch;
+    qk::DevBuf payv, hist;   // per-path payoffs and selection histograms of qk_price_distribution
     std::vector<qk_schedule_entry> schedule;   // lanes per thread by (precision, scheme, n_lanes)
     ~qk_ctx();
 };

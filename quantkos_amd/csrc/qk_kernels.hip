@@ -38,7 +38,7 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
                                          uint32_t key1, Real itm_eps, int64_t tile, Real *__restrict__ store,
                                          double *__restrict__ part_sum, double *__restrict__ part_m2,
                                          uint32_t *__restrict__ part_itm, int64_t out_index, int64_t lane_stride,
-                                         int64_t pay_stride) {
+                                         int64_t pay_stride, Real *__restrict__ pay_out = nullptr) {
     using M = Math<Real>;
     const int lid = threadIdx.x & (TILE - 1);
     const int64_t path = tile * TILE + lid;
@@ -92,6 +92,7 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
         const Real ST = PATHDEP ? S[l] : lc[l].S0 * M::exp(S[l]);
         for (int j = 0; j < n_pay; ++j) {
             const Real x = active ? payoff_eval<Real>(pays[j], ST, sm[l], mx[l], mn[l], n_steps) : Real(0);
+            if (pay_out != nullptr && l == 0 && j == 0 && active) pay_out[path] = x;
             const double xd = (double)x;
             const double s = wave_sum(xd);
             const double mean = s / n_act;
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
     run_tile<Real, MODEL, SCHEME, PATHDEP, STORE, L>(
         a.lanes + lbase, a.n_lanes - lbase, a.pays, a.n_pay, a.n_steps, a.n_paths, a.path0, a.key0, a.key1,
         a.itm_eps, tile, a.store, a.part_sum, a.part_m2, a.part_itm, (int64_t)lbase * a.n_pay * a.n_tiles + tile,
-        (int64_t)a.n_pay * a.n_tiles, a.n_tiles);
+        (int64_t)a.n_pay * a.n_tiles, a.n_tiles, lbase == 0 ? a.pay_out : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -571,6 +572,58 @@ hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_gr
     hipLaunchKernelGGL(k_greeks, dim3(1), dim3(64), 0, s, st, n_pay, m, out);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Order statistics of per-path values (payoff distribution percentiles) by
+// successive 8-bit histograms: each pass counts, for every still-open prefix
+// (the key bits fixed so far for one or more wanted ranks), the next 8 bits of
+// the keys that carry that prefix.  Counts are integers, so the selected
+// values are exact and independent of the launch shape.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t order_key(float x) {
+    const uint32_t b = __float_as_uint(x);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t order_key(double x) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_select_hist(const Real *__restrict__ vals, int64_t n, SelectArgs a,
+                                                     uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[SELECT_MAX_GROUPS * 256];
+    for (int i = threadIdx.x; i < a.n_groups * 256; i += 256) h[i] = 0u;
+    __syncthreads();
+    constexpr int BITS = 8 * (int)sizeof(Real);
+    const int hs = a.shift + 8;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const uint64_t k = (uint64_t)order_key(vals[i]);
+        const uint64_t hi = hs >= BITS ? 0ull : (k >> hs);
+        const uint32_t d = (uint32_t)(k >> a.shift) & 255u;
+        for (int g = 0; g < a.n_groups; ++g) {
+            if (hi == a.prefix[g]) {
+                atomicAdd(&h[g * 256 + d], 1u);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.n_groups * 256; i += 256)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+template <typename Real>
+hipError_t launch_select_hist(const Real *vals, int64_t n, const SelectArgs &a, uint32_t *hist, int blocks,
+                              hipStream_t s) {
+    if (n <= 0 || a.n_groups <= 0 || a.n_groups > SELECT_MAX_GROUPS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_select_hist<Real>, dim3(blocks), dim3(256), 0, s, vals, n, a, hist);
+    return hipGetLastError();
+}
+template hipError_t launch_select_hist<float>(const float *, int64_t, const SelectArgs &, uint32_t *, int,
+                                              hipStream_t);
+template hipError_t launch_select_hist<double>(const double *, int64_t, const SelectArgs &, uint32_t *, int,
+                                               hipStream_t);
 
 int pump_blocks(int64_t n_paths, int64_t *chunk) {
     const int64_t nb = std::min<int64_t>((n_paths + 255) / 256, PUMP_MAX_BLOCKS);

@@ -35,6 +35,17 @@ struct PumpArgs {
     int32_t block_start[PUMP_MAX_REQ + 1];
     int32_t n_req;
 };
+// One pass of the order-statistic selection (k_select_hist): the open key
+// prefixes and the bit position of the 8-bit digit counted under each.
+constexpr int SELECT_MAX_GROUPS = 16;
+struct SelectArgs {
+    uint64_t prefix[SELECT_MAX_GROUPS];
+    int32_t n_groups, shift;
+};
+template <typename Real>
+hipError_t launch_select_hist(const Real *vals, int64_t n, const SelectArgs &a, uint32_t *hist, int blocks,
+                              hipStream_t s);
+
 // Blocks and paths per block of one request's pump work.
 int pump_blocks(int64_t n_paths, int64_t *chunk);
 

@@ -52,6 +52,7 @@ struct PathArgs {
     double *part_m2;
     uint32_t *part_itm;
     Real *store;                // [step][path] or null
+    Real *pay_out;              // [path]: payoff 0 of lane 0 per path, or null
 };
 
 }  // namespace qk

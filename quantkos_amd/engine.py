@@ -9,8 +9,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (Bumps, Contract, Greeks, Payoff, Result, ScheduleEntry, ServerConfig, ServerStats, Stats,
-                   Timing)
+from ._lib import (Bumps, Contract, Distribution, Greeks, Payoff, Result, ScheduleEntry, ServerConfig, ServerStats,
+                   Stats, Timing)
 
 
 class QKError(RuntimeError):
@@ -148,6 +148,13 @@ class Engine:
         ptr = co.ctypes.data_as(C.POINTER(C.c_double)) if want_coeffs else None
         self._check(self.L.qk_price_american(self.h, C.byref(c), right, K, C.byref(out), ptr))
         return (out, co) if want_coeffs else out
+
+    def price_distribution(self, c: Contract, p: Payoff):
+        """(Stats, Distribution): the price and the discounted payoff percentiles
+        of the same paths; p.type AMERICAN prices by Longstaff-Schwartz."""
+        out, d = Stats(), Distribution()
+        self._check(self.L.qk_price_distribution(self.h, C.byref(c), C.byref(p), C.byref(out), C.byref(d)))
+        return out, d
 
     def simulate_paths(self, c: Contract, path0: int = 0) -> np.ndarray:
         """Spot paths [path0, path0 + c.n_paths) laid out [step][path]."""

@@ -139,6 +139,24 @@ void OptionPricer::evaluate_option_price() {
     metrics.option_price = stats.price;
     metrics.standard_error = stats.se;
     metrics.prob_itm = stats.prob_itm;
+    if (config.mc.analyze_risk_neutral_payoff_distribution) {
+        if (!has_dist) throw std::runtime_error("Distribution analysis requested, but payoffs array is empty.");
+        metrics.p0 = dist.p0;
+        metrics.p1 = dist.p1;
+        metrics.p5 = dist.p5;
+        metrics.p10 = dist.p10;
+        metrics.p20 = dist.p20;
+        metrics.p30 = dist.p30;
+        metrics.p40 = dist.p40;
+        metrics.median = dist.median;
+        metrics.p60 = dist.p60;
+        metrics.p70 = dist.p70;
+        metrics.p80 = dist.p80;
+        metrics.p90 = dist.p90;
+        metrics.p95 = dist.p95;
+        metrics.p99 = dist.p99;
+        metrics.p100 = dist.p100;
+    }
     metrics.is_computed = true;
 }
 
@@ -154,9 +172,29 @@ void OptionPricer::print_info_option_price() const {
               << std::fixed << std::setprecision(6)
               << "   Estimated Option Price   :  " << metrics.option_price << "\n"
               << "   Statistical Std Error    :  " << metrics.standard_error << "\n"
-              << "   Probability of ITM       :  " << metrics.prob_itm * 100.0 << " %\n"
-              << "  ========================================================\n"
-              << std::endl;
+              << "   Probability of ITM       :  " << metrics.prob_itm * 100.0 << " %\n";
+    if (config.mc.analyze_risk_neutral_payoff_distribution) {
+        std::cout << "  --------------------------------------------------------\n"
+                  << "   [Risk-Neutral Discounted Payout Distribution]\n"
+                  << "   Min  (Lower)             :  " << metrics.p0 << "\n"
+                  << "   1st  Percentile          :  " << metrics.p1 << "\n"
+                  << "   5th  Percentile          :  " << metrics.p5 << "\n"
+                  << "   10th Percentile          :  " << metrics.p10 << "\n"
+                  << "   20th Percentile          :  " << metrics.p20 << "\n"
+                  << "   30th Percentile          :  " << metrics.p30 << "\n"
+                  << "   40th Percentile          :  " << metrics.p40 << "\n"
+                  << "   50th Percentile (Median) :  " << metrics.median << "\n"
+                  << "   60th Percentile          :  " << metrics.p60 << "\n"
+                  << "   70th Percentile          :  " << metrics.p70 << "\n"
+                  << "   80th Percentile          :  " << metrics.p80 << "\n"
+                  << "   90th Percentile          :  " << metrics.p90 << "\n"
+                  << "   95th Percentile          :  " << metrics.p95 << "\n"
+                  << "   99th Percentile          :  " << metrics.p99 << "\n"
+                  << "   Max  (Upper)             :  " << metrics.p100 << "\n"
+                  << "  --------------------------------------------------------\n"
+                  << "   Time Spent to Sorting the Payoffs   :  " << dist.select_s << "s \n";
+    }
+    std::cout << "  ========================================================\n" << std::endl;
     std::cout.copyfmt(old);
 }
 
@@ -173,8 +211,14 @@ OptionPricer::MCOpPrices price_forward(const KC::UInputs &scaled, bool print) {
     const qk_contract c = to_contract(scaled);
     const qk_payoff p = to_payoff(scaled);
     qk_stats s{};
-    check(qk_price(ctx, &c, &p, &s), "qk_price");
     OptionPricer pr(scaled);
+    if (scaled.mc.analyze_risk_neutral_payoff_distribution) {
+        qk_distribution d{};
+        check(qk_price_distribution(ctx, &c, &p, &s, &d), "qk_price_distribution");
+        pr.accumulate_device_distribution(d);
+    } else {
+        check(qk_price(ctx, &c, &p, &s), "qk_price");
+    }
     pr.accumulate_device_stats(s);
     pr.evaluate_option_price();
     if (print) pr.print_info_option_price();
@@ -186,8 +230,15 @@ OptionPricer::MCOpPrices price_american(const KC::UInputs &scaled, bool print) {
     const qk_contract c = to_contract(scaled);
     const int right = scaled.options.opt_right == KI::OptRight::Call ? QK_CALL : QK_PUT;
     qk_stats s{};
-    check(qk_price_american(ctx, &c, right, scaled.options.StrikePrice, &s, nullptr), "qk_price_american");
     OptionPricer pr(scaled);
+    if (scaled.mc.analyze_risk_neutral_payoff_distribution) {
+        const qk_payoff p{QK_AMERICAN, right, scaled.options.StrikePrice, 0.0};
+        qk_distribution d{};
+        check(qk_price_distribution(ctx, &c, &p, &s, &d), "qk_price_distribution");
+        pr.accumulate_device_distribution(d);
+    } else {
+        check(qk_price_american(ctx, &c, right, scaled.options.StrikePrice, &s, nullptr), "qk_price_american");
+    }
     pr.accumulate_device_stats(s);
     pr.evaluate_option_price();
     if (print) pr.print_info_option_price();

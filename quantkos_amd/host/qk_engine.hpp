@@ -57,14 +57,21 @@ class OptionPricer {
 public:
     struct MCOpPrices {
         double option_price = 0.0, standard_error = 0.0, prob_itm = 0.0;
-        double p0 = 0, p1 = 0, p5 = 0, p10 = 0, p20 = 0, p30 = 0, p40 = 0, median = 0, p60 = 0, p70 = 0, p80 = 0,
-               p90 = 0, p95 = 0, p99 = 0, p100 = 0;
+        double p0 = -999., p1 = -999., p5 = -999., p10 = -999., p20 = -999., p30 = -999., p40 = -999.,
+               median = -999., p60 = -999., p70 = -999., p80 = -999., p90 = -999., p95 = -999., p99 = -999.,
+               p100 = -999.;
         bool is_computed = false;
     };
     explicit OptionPricer(const KC::UInputs &config) : config(config) {}
     // Takes the device's merged statistics of the whole path set (the
     // reference merges batch by batch, OptionPricer.hpp:123-193).
     void accumulate_device_stats(const qk_stats &s) { stats = s; }
+    // The device-selected distribution of the same paths (analyze_risk_neutral_
+    // discounted_payoff_distribution, OptionPricer.hpp:340-360).
+    void accumulate_device_distribution(const qk_distribution &d) {
+        dist = d;
+        has_dist = true;
+    }
     void evaluate_option_price();
     void print_info_option_price() const;
     const MCOpPrices &get_option_price_data() const;
@@ -72,6 +79,8 @@ public:
 private:
     const KC::UInputs &config;
     qk_stats stats{};
+    qk_distribution dist{};
+    bool has_dist = false;
     MCOpPrices metrics;
 };
 using MCOpPrices = OptionPricer::MCOpPrices;

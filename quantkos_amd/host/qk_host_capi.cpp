@@ -84,6 +84,9 @@ extern "C" qk_status qk_run_config(const char *yaml_path, int device, int32_t qu
         out->greeks = qk_greeks{r.Greeks.delta, r.Greeks.gamma, r.Greeks.vega, r.Greeks.vomma, r.Greeks.vanna,
                                 r.Greeks.rho, r.Greeks.theta, r.OptionPrice.option_price,
                                 r.OptionPrice.standard_error, r.n_greek_lanes};
+        const auto &o = r.OptionPrice;
+        out->dist = qk_distribution{o.p0,  o.p1,  o.p5,  o.p10, o.p20, o.p30, o.p40,  o.median,
+                                    o.p60, o.p70, o.p80, o.p90, o.p95, o.p99, o.p100, 0.0};
         out->t_total_s = r.Timing.s_elapsed_total;
         out->t_pricing_s = r.Timing.s_elapsed_for_pricing;
         out->t_greeks_s = r.Timing.s_elapsed_for_greeks;

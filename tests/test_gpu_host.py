@@ -110,3 +110,21 @@ def test_cli_prints_reference_report(in_tmp):
     assert "Estimated Option Price   :  5.1" in r.stdout
     assert "Total time spent for the computations" in r.stdout
     assert '"price": 5.1' in r.stdout
+
+
+def test_yaml_payoff_distribution(in_tmp):
+    """MC.analyze_risk_neutral_payoff_distribution: true fills MCOpPrices p0 ... p100
+    (OptionPricer.hpp:340-360) from the device selection; false leaves -999."""
+    text = open(os.path.join(GOLDEN, "bench_asian_qe.yaml")).read()
+    text = text.replace("N_Paths: 5000000", "N_Paths: 200000")
+    on = os.path.join(in_tmp, "dist_on.yaml")
+    off = os.path.join(in_tmp, "dist_off.yaml")
+    open(on, "w").write(text.replace("analyze_risk_neutral_payoff_distribution: false",
+                                     "analyze_risk_neutral_payoff_distribution: true"))
+    open(off, "w").write(text)
+    r = run_config(on)
+    d = r.dist
+    vals = [d.p0, d.p1, d.p5, d.p10, d.p20, d.p30, d.p40, d.median, d.p60, d.p70, d.p80, d.p90, d.p95, d.p99,
+            d.p100]
+    assert vals[0] == 0.0 and all(a <= b for a, b in zip(vals, vals[1:])) and vals[-1] > r.stats.price
+    assert run_config(off).dist.median == -999.0
