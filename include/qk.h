@@ -151,6 +151,19 @@ qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk_payoff *pa
 qk_status qk_price_american(qk_ctx *ctx, const qk_contract *c, int32_t right, double K,
                             qk_stats *out, double *coeffs_out);
 
+/* Regression basis of Longstaff-Schwartz.  The reference fixes LaguerreP03 at
+ * compile time (LSMAmerican.hpp:217-233, LSRegressionBasis in
+ * ConfigFileEnums.hpp:107-112, same order); weighted multiplies every basis
+ * function by exp(-X/2), the weighted Laguerre functions of Longstaff and
+ * Schwartz (2001), which the reference does not offer. */
+enum { QK_LAGUERRE_P02 = 0, QK_LAGUERRE_P03 = 1, QK_LAGUERRE_P04 = 2 };
+typedef struct { int32_t basis, weighted; } qk_lsm_options;
+
+/* qk_price_american with another regression basis; coeffs_out, if not NULL,
+ * receives [n_steps][n_basis] (n_basis = 3, 4, 5 for P02, P03, P04). */
+qk_status qk_price_american_ex(qk_ctx *ctx, const qk_contract *c, int32_t right, double K,
+                               const qk_lsm_options *opt, qk_stats *out, double *coeffs_out);
+
 /* Discounted payoff distribution (OptionPricer::analyze_risk_neutral_discounted_
  * payoff_distribution, OptionPricer.hpp:340-360, enabled by the YAML key
  * MC.analyze_risk_neutral_payoff_distribution): the per-path payoffs sorted
@@ -224,6 +237,8 @@ typedef struct {
     qk_bumps bumps;           /* the Greeks requested (all switches 0 if none)      */
     int32_t compute_greeks;
     int32_t write_paths;      /* Output.Name_Paths_Out_File is not empty            */
+    qk_lsm_options lsm;       /* MC.LSM_Basis, MC.LSM_Weighted_Basis (American)     */
+    int32_t analyze_distribution;   /* MC.analyze_risk_neutral_payoff_distribution  */
 } qk_run_spec;
 
 /* Config::Parser::parse (IO/InputParser.hpp:56, src/IO/ConfigParser.cpp:36-294)

@@ -84,6 +84,10 @@ def lib():
         L.qko_chan_merge.argtypes = [P(d), P(d), P(d), d, d, d]
         L.qko_lsm_solve.argtypes = [P(d), P(d)]
         L.qko_laguerre4.argtypes = [d, P(d)]
+        L.qko_lsm_solve_n.argtypes = [P(d), C.c_int32, P(d)]
+        L.qko_basis.argtypes = [d, C.c_int32, C.c_int32, P(d)]
+        L.qko_price_american_ex.argtypes = [P(Model), C.c_int64, C.c_int32, d, C.c_int32, C.c_int32, d,
+                                            P(Stats), P(d), P(d)]
         L.qko_gl64.argtypes = [P(d), P(d)]
         L.qko_heston_call.argtypes = [d] * 10
         L.qko_heston_call.restype = d
@@ -154,14 +158,16 @@ def price(m: Model, n_paths: int, payoffs, price_scale: float = 1.0):
 
 
 def price_american(m: Model, n_paths: int, right: int, K: float, price_scale: float = 1.0,
-                   want_coeffs=False, want_cf=False):
-    """Stats; with want_coeffs also the coefficients [n_steps][4]; with want_cf
-    also the final discounted cash flows per path (after the day-0 rule)."""
+                   want_coeffs=False, want_cf=False, n_basis=4, weighted=False):
+    """Stats; with want_coeffs also the coefficients [n_steps][n_basis]; with
+    want_cf also the final discounted cash flows per path (after the day-0
+    rule).  n_basis 3/4/5 = LaguerreP02/P03/P04; weighted multiplies the basis
+    by exp(-X/2)."""
     out = Stats()
-    co = np.zeros((m.n_steps, 4)) if want_coeffs else None
+    co = np.zeros((m.n_steps, n_basis)) if want_coeffs else None
     cf = np.zeros(n_paths) if want_cf else None
-    rc = lib().qko_price_american(C.byref(m), n_paths, right, K, price_scale, C.byref(out),
-                                  _dptr(co), _dptr(cf))
+    rc = lib().qko_price_american_ex(C.byref(m), n_paths, right, K, n_basis, int(bool(weighted)), price_scale,
+                                     C.byref(out), _dptr(co), _dptr(cf))
     if rc != 0:
         raise ValueError("qko_price_american failed")
     res = (out,) + ((co,) if want_coeffs else ()) + ((cf,) if want_cf else ())
@@ -195,6 +201,20 @@ def lsm_solve(sums):
     s = np.ascontiguousarray(sums, dtype=np.float64)
     c = np.zeros(4)
     fb = lib().qko_lsm_solve(_dptr(s), _dptr(c))
+    return c, bool(fb)
+
+
+def basis(X, n_basis=4, weighted=False):
+    L = np.zeros(n_basis)
+    lib().qko_basis(X, n_basis, int(bool(weighted)), _dptr(L))
+    return L
+
+
+def lsm_solve_n(sums, n_basis):
+    """sums = [count, M (n_basis x n_basis, row-major), B (n_basis)] -> (C, fallback)."""
+    s = np.ascontiguousarray(sums, dtype=np.float64)
+    c = np.zeros(n_basis)
+    fb = lib().qko_lsm_solve_n(_dptr(s), n_basis, _dptr(c))
     return c, bool(fb)
 
 

@@ -388,35 +388,46 @@ int qko_price(const qko_model *m, int64_t n_paths, const qko_payoff *pay, int32_
 /* ------------------------------------------------------------------------ */
 /* Longstaff-Schwartz (LSMAmerican.hpp, ForwardBackwardMC.hpp:204-255).     */
 /* ------------------------------------------------------------------------ */
-void qko_laguerre4(double X, double L[4]) {
-    /* Unweighted Laguerre polynomials in Horner form, LSMAmerican.hpp:172-198. */
+void qko_basis(double X, int32_t nb, int32_t weighted, double *L) {
+    /* Laguerre polynomials L0..L(nb-1) in Horner form, LSMAmerican.hpp:172-198
+     * (nb = 3, 4, 5 for the reference's LaguerreP02/P03/P04).  weighted: times
+     * exp(-X/2), the weighted Laguerre functions of Longstaff & Schwartz (2001),
+     * which the reference does not offer. */
     L[0] = 1.0;
-    L[1] = 1.0 - X;
-    L[2] = ((0.5 * X - 2.0) * X) + 1.0;
-    L[3] = (((-1.0 / 6.0 * X + 1.5) * X - 3.0) * X) + 1.0;
+    if (nb > 1) L[1] = 1.0 - X;
+    if (nb > 2) L[2] = ((0.5 * X - 2.0) * X) + 1.0;
+    if (nb > 3) L[3] = (((-1.0 / 6.0 * X + 1.5) * X - 3.0) * X) + 1.0;
+    if (nb > 4) L[4] = ((((1.0 / 24.0 * X - 2.0 / 3.0) * X + 3.0) * X - 4.0) * X) + 1.0;
+    if (weighted) {
+        const double w = exp(-0.5 * X);
+        for (int k = 0; k < nb; ++k) L[k] *= w;
+    }
 }
 
+void qko_laguerre4(double X, double L[4]) { qko_basis(X, 4, 0, L); }
+
 /* solve_system_on_host_cholesky (LSMAmerican.hpp:516-581) and
- * compute_fallback_coefficients (:605-613). */
-int qko_lsm_solve(const double sums[21], double C[4]) {
+ * compute_fallback_coefficients (:605-613) for nb basis functions;
+ * sums = {count, M[nb*nb] row-major, B[nb]}. */
+int qko_lsm_solve_n(const double *sums, int32_t nb, double *C) {
     const double count = sums[0];
-    const double *Ms = sums + 1, *Bs = sums + 17;
-    for (int i = 0; i < 4; ++i) C[i] = 0.0;
-    if (count < 4.0) {
+    const double *Ms = sums + 1, *Bs = sums + 1 + nb * nb;
+    for (int i = 0; i < nb; ++i) C[i] = 0.0;
+    if (count < (double)nb) {
         if (count > 0.0) C[0] = Bs[0] / count;
         return 1;
     }
-    double M[4][4], L[4][4] = {{0}};
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) M[i][j] = Ms[i * 4 + j] + (i == j ? 1e-8 : 0.0);
-    for (int i = 0; i < 4; ++i) {
+    double M[QKO_MAX_BASIS][QKO_MAX_BASIS], L[QKO_MAX_BASIS][QKO_MAX_BASIS] = {{0}};
+    for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j) M[i][j] = Ms[i * nb + j] + (i == j ? 1e-8 : 0.0);
+    for (int i = 0; i < nb; ++i) {
         for (int j = 0; j <= i; ++j) {
             double s = 0.0;
             for (int k = 0; k < j; ++k) s += L[i][k] * L[j][k];
             if (i == j) {
                 const double val = M[i][i] - s;
                 if (val < 1e-14) {
-                    for (int q = 0; q < 4; ++q) C[q] = 0.0;
+                    for (int q = 0; q < nb; ++q) C[q] = 0.0;
                     C[0] = Bs[0] / count;
                     return 1;
                 }
@@ -426,57 +437,67 @@ int qko_lsm_solve(const double sums[21], double C[4]) {
             }
         }
     }
-    double y[4];
-    for (int i = 0; i < 4; ++i) {
+    double y[QKO_MAX_BASIS];
+    for (int i = 0; i < nb; ++i) {
         double s = 0.0;
         for (int k = 0; k < i; ++k) s += L[i][k] * y[k];
         y[i] = (Bs[i] - s) / L[i][i];
     }
-    for (int i = 3; i >= 0; --i) {
+    for (int i = nb - 1; i >= 0; --i) {
         double s = 0.0;
-        for (int k = i + 1; k < 4; ++k) s += L[k][i] * C[k];
+        for (int k = i + 1; k < nb; ++k) s += L[k][i] * C[k];
         C[i] = (y[i] - s) / L[i][i];
     }
     return 0;
 }
 
+int qko_lsm_solve(const double sums[21], double C[4]) { return qko_lsm_solve_n(sums, 4, C); }
+
 int qko_price_american(const qko_model *m, int64_t n_paths, int32_t right, double K,
                        double price_scale, qko_stats *out, double *coeffs_out, double *cf_out) {
-    if (!m || !out || n_paths < 1 || m->n_steps < 2) return -1;
+    return qko_price_american_ex(m, n_paths, right, K, 4, 0, price_scale, out, coeffs_out, cf_out);
+}
+
+int qko_price_american_ex(const qko_model *m, int64_t n_paths, int32_t right, double K, int32_t nb,
+                          int32_t weighted, double price_scale, qko_stats *out, double *coeffs_out,
+                          double *cf_out) {
+    if (!m || !out || n_paths < 1 || m->n_steps < 2 || nb < 3 || nb > QKO_MAX_BASIS) return -1;
     const int32_t N = m->n_steps;
     double *S = malloc(sizeof(double) * n_paths * (size_t)N);
     double *cf = malloc(sizeof(double) * n_paths);
     if (!S || !cf) { free(S); free(cf); return -2; }
     qko_simulate(m, 0, n_paths, S, NULL, NULL, NULL, NULL, NULL);
     const double disc = exp(-m->r * (m->T / (double)N));
+    const int ns = 1 + nb * nb + nb;
     for (int64_t i = 0; i < n_paths; ++i) cf[i] = vanilla(right, S[(int64_t)(N - 1) * n_paths + i], K);
     for (int32_t t = N - 2; t >= 0; --t) {
         const double *St = S + (int64_t)t * n_paths;
-        double sums[21] = {0};
+        double sums[1 + QKO_MAX_BASIS * QKO_MAX_BASIS + QKO_MAX_BASIS] = {0};
         for (int64_t i = 0; i < n_paths; ++i) {          /* LSMAmerican.hpp:443-471 */
             const double intrinsic = vanilla(right, St[i], K);
             if (intrinsic > 0.0) {
-                double L[4];
-                qko_laguerre4(St[i] / K, L);
+                double L[QKO_MAX_BASIS];
+                qko_basis(St[i] / K, nb, weighted, L);
                 const double Y = cf[i] * disc;
-                for (int a = 0; a < 4; ++a) {
-                    sums[17 + a] += L[a] * Y;
-                    for (int b = 0; b < 4; ++b) sums[1 + a * 4 + b] += L[a] * L[b];
+                for (int a = 0; a < nb; ++a) {
+                    sums[1 + nb * nb + a] += L[a] * Y;
+                    for (int b = 0; b < nb; ++b) sums[1 + a * nb + b] += L[a] * L[b];
                 }
                 sums[0] += 1.0;
             }
         }
-        double C[4];
-        qko_lsm_solve(sums, C);
-        if (coeffs_out) memcpy(coeffs_out + (int64_t)t * 4, C, sizeof(C));
+        (void)ns;
+        double C[QKO_MAX_BASIS];
+        qko_lsm_solve_n(sums, nb, C);
+        if (coeffs_out) memcpy(coeffs_out + (int64_t)t * nb, C, sizeof(double) * (size_t)nb);
         for (int64_t i = 0; i < n_paths; ++i) {          /* LSMAmerican.hpp:318-339 */
             const double intrinsic = vanilla(right, St[i], K);
             const double dcf = cf[i] * disc;
             if (intrinsic > 0.0) {
-                double L[4];
-                qko_laguerre4(St[i] / K, L);
+                double L[QKO_MAX_BASIS];
+                qko_basis(St[i] / K, nb, weighted, L);
                 double cont = 0.0;
-                for (int a = 0; a < 4; ++a) cont += C[a] * L[a];
+                for (int a = 0; a < nb; ++a) cont += C[a] * L[a];
                 cf[i] = intrinsic > cont ? intrinsic : dcf;
             } else {
                 cf[i] = dcf;

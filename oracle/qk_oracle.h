@@ -96,6 +96,12 @@ int qko_price(const qko_model *m, int64_t n_paths, const qko_payoff *pay, int32_
               double price_scale, qko_stats *out);
 int qko_price_american(const qko_model *m, int64_t n_paths, int32_t right, double K,
                        double price_scale, qko_stats *out, double *coeffs_out, double *cf_out);
+/* nb = 3, 4, 5 basis functions (LaguerreP02/P03/P04); weighted: times exp(-X/2).
+ * coeffs_out: [n_steps][nb]. */
+#define QKO_MAX_BASIS 5
+int qko_price_american_ex(const qko_model *m, int64_t n_paths, int32_t right, double K, int32_t nb,
+                          int32_t weighted, double price_scale, qko_stats *out, double *coeffs_out,
+                          double *cf_out);
 int qko_greeks_eval(const qko_model *m, int64_t n_paths, const qko_payoff *pay,
                     const qko_bumps *b, double price_scale, qko_greeks *out);
 
@@ -105,6 +111,9 @@ void qko_chan_merge(double *n_a, double *mean_a, double *m2_a,
 /* sums = {count, M[16] row-major, B[4]} ; writes 4 coefficients. Returns 1 on fallback. */
 int qko_lsm_solve(const double sums[21], double coeffs[4]);
 void qko_laguerre4(double X, double L[4]);
+/* sums = {count, M[nb*nb] row-major, B[nb]} ; writes nb coefficients. */
+int qko_lsm_solve_n(const double *sums, int32_t nb, double *coeffs);
+void qko_basis(double X, int32_t nb, int32_t weighted, double *L);
 
 /* ---- closed forms ---- */
 void qko_gl64(double x[64], double w[64]);

@@ -18,6 +18,7 @@ EULER, MILSTEIN, QE = 0, 1, 2
  LOOKBACK_FLOATING, LOOKBACK_FIXED, BINARY_CASH, BINARY_ASSET, AMERICAN) = range(11)
 CALL, PUT = 0, 1
 FP32, FP64 = 0, 1
+LAGUERRE_P02, LAGUERRE_P03, LAGUERRE_P04 = 0, 1, 2
 
 
 class Contract(C.Structure):
@@ -63,9 +64,14 @@ class Distribution(C.Structure):
                                           "p80", "p90", "p95", "p99", "p100", "select_s")]
 
 
+class LsmOptions(C.Structure):
+    _fields_ = [("basis", C.c_int32), ("weighted", C.c_int32)]
+
+
 class RunSpec(C.Structure):
     _fields_ = [("contract", Contract), ("payoff", Payoff), ("bumps", Bumps),
-                ("compute_greeks", C.c_int32), ("write_paths", C.c_int32)]
+                ("compute_greeks", C.c_int32), ("write_paths", C.c_int32), ("lsm", LsmOptions),
+                ("analyze_distribution", C.c_int32)]
 
 
 class RunResult(C.Structure):
@@ -127,6 +133,8 @@ def load():
         "qk_price_american": ([vp, P(Contract), C.c_int32, C.c_double, P(Stats), P(C.c_double)], st),
         "qk_simulate_paths": ([vp, P(Contract), C.c_int64, P(C.c_double)], st),
         "qk_price_distribution": ([vp, P(Contract), P(Payoff), P(Stats), P(Distribution)], st),
+        "qk_price_american_ex": ([vp, P(Contract), C.c_int32, C.c_double, P(LsmOptions), P(Stats),
+                                  P(C.c_double)], st),
         "qk_server_create": ([vp, P(ServerConfig), P(vp)], st),
         "qk_server_destroy": ([vp], st),
         "qk_server_submit": ([vp, P(Contract), P(Payoff), P(C.c_uint64)], st),

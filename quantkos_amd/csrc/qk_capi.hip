@@ -196,7 +196,7 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
         b += (size_t)c.n_paths * (size_t)c.n_steps * rs;   // path store [date][path]
         b += (size_t)c.n_paths * rs;                         // cash flows
         int64_t chunk = 0;
-        b += (size_t)c.n_steps * 4 * 8 + (size_t)pump_blocks(c.n_paths, &chunk) * 16 * 8;
+        b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 + (size_t)pump_blocks(c.n_paths, &chunk) * LSM_STRIDE * 8;
     }
     return b;
 }
@@ -539,7 +539,7 @@ static qk_status generate_store(qk_ctx *ctx, const qk_contract &c, int64_t path0
 
 template <typename Real>
 static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, double K, qk_stats *out,
-                              double *coeffs_out) {
+                              double *coeffs_out, int n_basis = 4, bool weighted = false) {
     qk_status st = generate_store<Real>(ctx, c);
     if (st != QK_OK) return st;
     const int64_t n = c.n_paths;
@@ -548,8 +548,8 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     int64_t chunk = 0;
     const int nb = pump_blocks(n, &chunk);
     if ((st = ctx->cf.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
-    if ((st = ctx->coeffs.reserve(ctx, (size_t)N * 4 * 8)) != QK_OK) return st;
-    if ((st = ctx->bpart.reserve(ctx, (size_t)nb * 16 * 8)) != QK_OK) return st;
+    if ((st = ctx->coeffs.reserve(ctx, (size_t)N * n_basis * 8)) != QK_OK) return st;
+    if ((st = ctx->bpart.reserve(ctx, (size_t)nb * LSM_STRIDE * 8)) != QK_OK) return st;
     if ((st = ctx->psum.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pm2.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pitm.reserve(ctx, n_tiles * 4)) != QK_OK) return st;
@@ -561,7 +561,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     const double one = 1.0, scale = c.price_scale;
     QK_HIP(hipMemcpyAsync(ctx->disc.p, &one, 8, hipMemcpyHostToDevice, s));
     QK_HIP(hipMemcpyAsync(ctx->scale.p, &scale, 8, hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemsetAsync(ctx->coeffs.p, 0, (size_t)N * 4 * 8, s));
+    QK_HIP(hipMemsetAsync(ctx->coeffs.p, 0, (size_t)N * n_basis * 8, s));
     PumpArgs<Real> a{};
     PumpReq<Real> &q = a.r[0];
     q.store = (const Real *)ctx->store.p;
@@ -584,7 +584,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     a.block_start[1] = nb;
     for (int t = N - 1; t >= 0; --t) {                   // ForwardBackwardMC.hpp:219-242
         q.t = t;
-        QK_HIP(launch_pump<Real>(a, s));
+        QK_HIP(launch_pump<Real>(a, n_basis, weighted, s));
     }
     QK_HIP(hipEventRecord(ctx->ev[3], s));
     QK_HIP(launch_merge((const double *)ctx->psum.p, (const double *)ctx->pm2.p, (const uint32_t *)ctx->pitm.p,
@@ -592,7 +592,8 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
                         ctx->mscratch.p, (qk_stats *)ctx->stats.p, s));
     QK_HIP(hipEventRecord(ctx->ev[2], s));
     QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, s));
-    if (coeffs_out) QK_HIP(hipMemcpyAsync(coeffs_out, ctx->coeffs.p, (size_t)N * 4 * 8, hipMemcpyDeviceToHost, s));
+    if (coeffs_out)
+        QK_HIP(hipMemcpyAsync(coeffs_out, ctx->coeffs.p, (size_t)N * n_basis * 8, hipMemcpyDeviceToHost, s));
     QK_HIP(hipStreamSynchronize(s));
     // Immediate exercise check (LSMAmerican.hpp:409-418): if today's intrinsic
     // beats the mean discounted cash flow, every path exercises at t0.
@@ -753,6 +754,25 @@ extern "C" qk_status qk_price_distribution(qk_ctx *ctx, const qk_contract *c, co
     QK_HIP(hipSetDevice(ctx->device));
     return c->precision == QK_FP32 ? run_distribution<float>(ctx, *c, *p, out, dist)
                                    : run_distribution<double>(ctx, *c, *p, out, dist);
+}
+
+extern "C" qk_status qk_price_american_ex(qk_ctx *ctx, const qk_contract *c, int32_t right, double K,
+                                          const qk_lsm_options *opt, qk_stats *out, double *coeffs_out) {
+    if (!ctx || !out || !opt) return fail(ctx, QK_EINVAL, "null argument");
+    if (opt->basis < QK_LAGUERRE_P02 || opt->basis > QK_LAGUERRE_P04)
+        return fail(ctx, QK_EINVAL, "lsm basis must be LaguerreP02, P03 or P04");
+    qk_status st = validate_contract(ctx, c);
+    if (st != QK_OK) return st;
+    if (right != QK_CALL && right != QK_PUT) return fail(ctx, QK_EINVAL, "right out of range");
+    if (!(K > 0.0)) return fail(ctx, QK_EINVAL, "K must be positive");
+    if ((st = check_budget(ctx, footprint(*c, 1, 1, true))) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    const int nb = 3 + opt->basis;
+    const bool w = opt->weighted != 0;
+    st = c->precision == QK_FP32 ? run_american<float>(ctx, *c, right, K, out, coeffs_out, nb, w)
+                                 : run_american<double>(ctx, *c, right, K, out, coeffs_out, nb, w);
+    if (st != QK_OK) return st;
+    return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, true);
 }
 
 extern "C" qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, int64_t path0, double *S_out) {

@@ -233,37 +233,49 @@ __global__ void k_greeks(const qk_stats *__restrict__ st, int n_pay, GreekMap m,
 // ---------------------------------------------------------------------------
 // Longstaff-Schwartz.
 // ---------------------------------------------------------------------------
-template <typename Real>
-__device__ __forceinline__ void laguerre4(Real X, Real L[4]) {
-    // Unweighted Laguerre polynomials, Horner form (LSMAmerican.hpp:172-198).
+// Regression basis at X = S/K: Laguerre polynomials L0..L(NB-1) in Horner form
+// (LSMAmerican.hpp:172-198; NB = 3, 4, 5 are the reference's LaguerreP02/P03/
+// P04), times exp(-X/2) for the weighted Laguerre functions of Longstaff and
+// Schwartz (2001) when W.
+template <typename Real, int NB, bool W>
+__device__ __forceinline__ void lsm_basis(Real X, Real L[NB]) {
     L[0] = Real(1);
-    L[1] = Real(1) - X;
-    L[2] = ((Real(0.5) * X - Real(2)) * X) + Real(1);
-    L[3] = (((Real(-1.0 / 6.0) * X + Real(1.5)) * X - Real(3)) * X) + Real(1);
+    if constexpr (NB > 1) L[1] = Real(1) - X;
+    if constexpr (NB > 2) L[2] = ((Real(0.5) * X - Real(2)) * X) + Real(1);
+    if constexpr (NB > 3) L[3] = (((Real(-1.0 / 6.0) * X + Real(1.5)) * X - Real(3)) * X) + Real(1);
+    if constexpr (NB > 4)
+        L[4] = ((((Real(1.0 / 24.0) * X - Real(2.0 / 3.0)) * X + Real(3)) * X - Real(4)) * X) + Real(1);
+    if constexpr (W) {
+        const Real w = Math<Real>::exp(Real(-0.5) * X);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) L[k] *= w;
+    }
 }
 
 // Cholesky solve with the reference's 1e-8 ridge, 1e-14 pivot limit and
-// mean-payoff fallback (LSMAmerican.hpp:516-581, :605-613).  s = {count, M00,
-// M01, M02, M03, M11, M12, M13, M22, M23, M33, B0..B3}.
-__device__ void lsm_solve(const double s[15], double C[4]) {
+// mean-payoff fallback (LSMAmerican.hpp:516-581, :605-613).  s is laid out as
+// LsmSums<NB>: {count, upper triangle of M row by row, B}.
+template <int NB>
+__device__ void lsm_solve(const double *s, double C[NB]) {
+    using S = LsmSums<NB>;
     const double count = s[0];
-    C[0] = C[1] = C[2] = C[3] = 0.0;
-    if (count < 4.0) {
-        if (count > 0.0) C[0] = s[11] / count;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) C[k] = 0.0;
+    if (count < (double)NB) {
+        if (count > 0.0) C[0] = s[S::b(0)] / count;
         return;
     }
-    const int up[4][4] = {{1, 2, 3, 4}, {2, 5, 6, 7}, {3, 6, 8, 9}, {4, 7, 9, 10}};
-    double Lm[4][4] = {};
-    for (int i = 0; i < 4; ++i) {
+    double Lm[NB][NB] = {};
+    for (int i = 0; i < NB; ++i) {
         for (int j = 0; j <= i; ++j) {
             double acc = 0.0;
             for (int k = 0; k < j; ++k) acc += Lm[i][k] * Lm[j][k];
-            const double Mij = s[up[i][j]] + (i == j ? 1e-8 : 0.0);
+            const double Mij = s[S::m(j, i)] + (i == j ? 1e-8 : 0.0);
             if (i == j) {
                 const double val = Mij - acc;
                 if (val < 1e-14) {
-                    C[0] = s[11] / count;
-                    C[1] = C[2] = C[3] = 0.0;
+                    C[0] = s[S::b(0)] / count;
+                    for (int k = 1; k < NB; ++k) C[k] = 0.0;
                     return;
                 }
                 Lm[i][i] = sqrt(val);
@@ -272,15 +284,15 @@ __device__ void lsm_solve(const double s[15], double C[4]) {
             }
         }
     }
-    double y[4];
-    for (int i = 0; i < 4; ++i) {
+    double y[NB];
+    for (int i = 0; i < NB; ++i) {
         double acc = 0.0;
         for (int k = 0; k < i; ++k) acc += Lm[i][k] * y[k];
-        y[i] = (s[11 + i] - acc) / Lm[i][i];
+        y[i] = (s[S::b(i)] - acc) / Lm[i][i];
     }
-    for (int i = 3; i >= 0; --i) {
+    for (int i = NB - 1; i >= 0; --i) {
         double acc = 0.0;
-        for (int k = i + 1; k < 4; ++k) acc += Lm[k][i] * C[k];
+        for (int k = i + 1; k < NB; ++k) acc += Lm[k][i] * C[k];
         C[i] = (y[i] - acc) / Lm[i][i];
     }
 }
@@ -288,36 +300,37 @@ __device__ void lsm_solve(const double s[15], double C[4]) {
 // Regression step of one Longstaff-Schwartz date for every request listed in
 // the arguments, one block per request: sums the block partials of the
 // in-the-money Gram matrix / right-hand side of date t (written by the pump
-// step of date t+1) in block order and solves the 4x4 normal equations in FP64
-// into coeffs[t].  Requests at their last date have nothing to regress.
-template <typename Real>
+// step of date t+1) in block order and solves the NB x NB normal equations in
+// FP64 into coeffs[t].  Requests at their last date have nothing to regress.
+template <typename Real, int NB>
 __global__ __launch_bounds__(256) void k_lsm_coef(PumpArgs<Real> a) {
+    constexpr int NS = LsmSums<NB>::NS;
     const PumpReq<Real> &q = a.r[blockIdx.x];
     const int t = q.t;
     if (t >= q.n_steps - 1) return;
-    __shared__ double red[15][256];
-    double acc[15];
+    __shared__ double red[NS][256];
+    double acc[NS];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) acc[k] = 0.0;
+    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
     for (int j = threadIdx.x; j < q.nb; j += 256) {
 #pragma unroll
-        for (int k = 0; k < 15; ++k) acc[k] += q.bpart[(int64_t)j * 16 + k];
+        for (int k = 0; k < NS; ++k) acc[k] += q.bpart[(int64_t)j * LSM_STRIDE + k];
     }
 #pragma unroll
-    for (int k = 0; k < 15; ++k) red[k][threadIdx.x] = acc[k];
+    for (int k = 0; k < NS; ++k) red[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
 #pragma unroll
-            for (int k = 0; k < 15; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+            for (int k = 0; k < NS; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double sm[15], C[4];
-        for (int k = 0; k < 15; ++k) sm[k] = red[k][0];
-        lsm_solve(sm, C);
-        for (int k = 0; k < 4; ++k) q.coeffs[(int64_t)t * 4 + k] = C[k];
+        double sm[NS], C[NB];
+        for (int k = 0; k < NS; ++k) sm[k] = red[k][0];
+        lsm_solve<NB>(sm, C);
+        for (int k = 0; k < NB; ++k) q.coeffs[(int64_t)t * NB + k] = C[k];
     }
 }
 
@@ -328,22 +341,22 @@ __global__ __launch_bounds__(256) void k_lsm_coef(PumpArgs<Real> a) {
 // the in-the-money Gram sums of date t-1 in FP64 into its partial, and at date
 // 0 applies the final one-step discount and writes the per-tile moments of the
 // cash flows.
-template <typename Real>
+template <typename Real, int NB, bool W>
 __global__ __launch_bounds__(256) void k_lsm_pump(PumpArgs<Real> a) {
+    using S = LsmSums<NB>;
+    constexpr int NS = S::NS;
     int r = 0;
     while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
     const PumpReq<Real> &q = a.r[r];
     const int b = (int)blockIdx.x - a.block_start[r];
     const int t = q.t, N = q.n_steps;
     const int64_t n = q.n_paths;
-    double C[4] = {0.0, 0.0, 0.0, 0.0};
-    if (t < N - 1) {
+    double C[NB];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) C[k] = q.coeffs[(int64_t)t * 4 + k];
-    }
-    double g[15];
+    for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? q.coeffs[(int64_t)t * NB + k] : 0.0;
+    double g[NS];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) g[k] = 0.0;
+    for (int k = 0; k < NS; ++k) g[k] = 0.0;
     const int lid = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t p0 = (int64_t)b * q.chunk;
     const int64_t p1 = p0 + q.chunk < n ? p0 + q.chunk : n;
@@ -360,28 +373,30 @@ __global__ __launch_bounds__(256) void k_lsm_pump(PumpArgs<Real> a) {
                 const Real dcf = q.cf[i] * q.disc;
                 c = dcf;
                 if (intrinsic > Real(0)) {
-                    Real L[4];
-                    laguerre4<Real>(Math<Real>::div(S, q.K), L);
+                    Real L[NB];
+                    lsm_basis<Real, NB, W>(Math<Real>::div(S, q.K), L);
                     double cont = 0.0;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) cont += C[k] * (double)L[k];
+                    for (int k = 0; k < NB; ++k) cont += C[k] * (double)L[k];
                     if ((double)intrinsic > cont) c = intrinsic;
                 }
             }
             if (t >= 1) {
                 const Real S1 = q.store[(int64_t)(t - 1) * n + i];
                 if (vanilla<Real>(q.right, S1, q.K) > Real(0)) {
-                    Real Lr[4];
-                    laguerre4<Real>(Math<Real>::div(S1, q.K), Lr);
+                    Real Lr[NB];
+                    lsm_basis<Real, NB, W>(Math<Real>::div(S1, q.K), Lr);
                     const double Y = (double)(c * q.disc);
-                    double L[4];
+                    double L[NB];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) L[k] = (double)Lr[k];
+                    for (int k = 0; k < NB; ++k) L[k] = (double)Lr[k];
                     g[0] += 1.0;
-                    g[1] += L[0] * L[0]; g[2] += L[0] * L[1]; g[3] += L[0] * L[2]; g[4] += L[0] * L[3];
-                    g[5] += L[1] * L[1]; g[6] += L[1] * L[2]; g[7] += L[1] * L[3];
-                    g[8] += L[2] * L[2]; g[9] += L[2] * L[3]; g[10] += L[3] * L[3];
-                    g[11] += L[0] * Y; g[12] += L[1] * Y; g[13] += L[2] * Y; g[14] += L[3] * Y;
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+#pragma unroll
+                        for (int v = u; v < NB; ++v) g[S::m(u, v)] += L[u] * L[v];
+                        g[S::b(u)] += L[u] * Y;
+                    }
                 }
             } else {
                 c = c * q.disc;   // LSMAmerican.hpp:393-405
@@ -408,17 +423,17 @@ __global__ __launch_bounds__(256) void k_lsm_pump(PumpArgs<Real> a) {
         }
     }
     if (t >= 1) {
-        __shared__ double wpart[WAVES_PER_BLOCK][16];
+        __shared__ double wpart[WAVES_PER_BLOCK][NS];
 #pragma unroll
-        for (int k = 0; k < 15; ++k) {
+        for (int k = 0; k < NS; ++k) {
             const double s = wave_sum(g[k]);
             if (lid == 0) wpart[wv][k] = s;
         }
         __syncthreads();
-        if (threadIdx.x < 15) {
+        if ((int)threadIdx.x < NS) {
             double s = 0.0;
             for (int w = 0; w < WAVES_PER_BLOCK; ++w) s += wpart[w][threadIdx.x];
-            q.bpart[(int64_t)b * 16 + threadIdx.x] = s;
+            q.bpart[(int64_t)b * LSM_STRIDE + threadIdx.x] = s;
         }
     }
 }
@@ -632,11 +647,24 @@ int pump_blocks(int64_t n_paths, int64_t *chunk) {
     return (int)((n_paths + c - 1) / c);
 }
 
+template <typename Real, int NB, bool W>
+static void launch_pump_nb(const PumpArgs<Real> &a, hipStream_t s) {
+    hipLaunchKernelGGL((k_lsm_coef<Real, NB>), dim3(a.n_req), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_lsm_pump<Real, NB, W>), dim3(a.block_start[a.n_req]), dim3(256), 0, s, a);
+}
+
 template <typename Real>
-hipError_t launch_pump(const PumpArgs<Real> &a, hipStream_t s) {
+hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s) {
     if (a.n_req <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lsm_coef<Real>, dim3(a.n_req), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_lsm_pump<Real>, dim3(a.block_start[a.n_req]), dim3(256), 0, s, a);
+    switch (n_basis * 2 + (weighted ? 1 : 0)) {
+    case 6: launch_pump_nb<Real, 3, false>(a, s); break;
+    case 7: launch_pump_nb<Real, 3, true>(a, s); break;
+    case 8: launch_pump_nb<Real, 4, false>(a, s); break;
+    case 9: launch_pump_nb<Real, 4, true>(a, s); break;
+    case 10: launch_pump_nb<Real, 5, false>(a, s); break;
+    case 11: launch_pump_nb<Real, 5, true>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -661,7 +689,7 @@ template hipError_t launch_batch<double>(const ReqD<double> *, const ItemD *, in
                                          hipStream_t);
 template hipError_t launch_paths<float>(const PathArgs<float> &, int, int, bool, bool, int, hipStream_t);
 template hipError_t launch_paths<double>(const PathArgs<double> &, int, int, bool, bool, int, hipStream_t);
-template hipError_t launch_pump<float>(const PumpArgs<float> &, hipStream_t);
-template hipError_t launch_pump<double>(const PumpArgs<double> &, hipStream_t);
+template hipError_t launch_pump<float>(const PumpArgs<float> &, int, bool, hipStream_t);
+template hipError_t launch_pump<double>(const PumpArgs<double> &, int, bool, hipStream_t);
 
 }  // namespace qk

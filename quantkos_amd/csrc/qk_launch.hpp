@@ -14,6 +14,18 @@ struct GreekMap {
     double h_S_real, h_vol, h_r, h_T;
 };
 
+// Layout of one date's regression sums for NB basis functions: count, the
+// upper triangle of the Gram matrix row by row, the right-hand side.
+template <int NB>
+struct LsmSums {
+    static constexpr int NM = NB * (NB + 1) / 2;
+    static constexpr int NS = 1 + NM + NB;
+    __host__ __device__ static constexpr int m(int i, int j) { return 1 + i * NB - i * (i - 1) / 2 + (j - i); }
+    __host__ __device__ static constexpr int b(int k) { return 1 + NM + k; }
+};
+constexpr int LSM_MAX_BASIS = 5;
+constexpr int LSM_STRIDE = 24;          // doubles per block partial (>= LsmSums<5>::NS = 21)
+
 // One request of a Longstaff-Schwartz step (k_lsm_coef + k_lsm_pump).
 constexpr int PUMP_MAX_REQ = 24;        // requests per pump launch (kernel-argument size)
 constexpr int PUMP_MAX_BLOCKS = 1024;   // blocks per request per date
@@ -21,8 +33,8 @@ template <typename Real>
 struct PumpReq {
     const Real *store;    // [n_steps][n_paths]
     Real *cf;             // [n_paths]
-    double *bpart;        // [nb][16] Gram partials of the date below t
-    double *coeffs;       // [n_steps][4]
+    double *bpart;        // [nb][LSM_STRIDE] Gram partials of the date below t
+    double *coeffs;       // [n_steps][n_basis]
     double *psum, *pm2;   // tile moments of the final cash flows
     uint32_t *pitm;
     int64_t n_paths, chunk;
@@ -92,5 +104,5 @@ hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pit
                         hipStream_t s);
 hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s);
 template <typename Real>
-hipError_t launch_pump(const PumpArgs<Real> &a, hipStream_t s);
+hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s);
 }  // namespace qk

@@ -174,7 +174,7 @@ Layout layout_of(const qk_contract &c, bool american, int64_t n_tiles, int64_t n
         L.store = take((size_t)c.n_paths * c.n_steps * rs);
         L.cf = take((size_t)c.n_paths * rs);
         L.coeffs = take((size_t)c.n_steps * 4 * 8);
-        L.bpart = take((size_t)nb * 16 * 8);
+        L.bpart = take((size_t)nb * LSM_STRIDE * 8);
         L.mreq = take(sizeof(MergeReqD));
         L.creq = take(n_chunks * 4);
         L.stats = take(sizeof(qk_stats));
@@ -390,7 +390,7 @@ hipError_t launch_pump_step(qk_server *srv, bool fp32) {
         ++a.n_req;
     }
     a.block_start[a.n_req] = blocks;
-    return launch_pump<Real>(a, srv->pump_stream);
+    return launch_pump<Real>(a, 4, false, srv->pump_stream);   // LaguerreP03, the reference's basis
 }
 
 // Enqueues the merge of a request whose date-0 step was just queued.

@@ -142,11 +142,19 @@ class Engine:
         self._check(self.L.qk_price_greeks(self.h, C.byref(c), pa, len(payoffs), C.byref(b), out))
         return list(out)
 
-    def price_american(self, c: Contract, right: int, K: float, want_coeffs: bool = False):
+    def price_american(self, c: Contract, right: int, K: float, want_coeffs: bool = False,
+                       basis: int = _lib.LAGUERRE_P03, weighted: bool = False):
+        """Longstaff-Schwartz price; basis LAGUERRE_P02/P03/P04 (3/4/5 functions),
+        weighted multiplies them by exp(-X/2).  want_coeffs adds [n_steps][n_basis]."""
         out = Stats()
-        co = np.zeros((c.n_steps, 4)) if want_coeffs else None
+        nb = 3 + basis
+        co = np.zeros((c.n_steps, nb)) if want_coeffs else None
         ptr = co.ctypes.data_as(C.POINTER(C.c_double)) if want_coeffs else None
-        self._check(self.L.qk_price_american(self.h, C.byref(c), right, K, C.byref(out), ptr))
+        if basis == _lib.LAGUERRE_P03 and not weighted:
+            self._check(self.L.qk_price_american(self.h, C.byref(c), right, K, C.byref(out), ptr))
+        else:
+            opt = _lib.LsmOptions(basis, int(bool(weighted)))
+            self._check(self.L.qk_price_american_ex(self.h, C.byref(c), right, K, C.byref(opt), C.byref(out), ptr))
         return (out, co) if want_coeffs else out
 
     def price_distribution(self, c: Contract, p: Payoff):

@@ -52,6 +52,13 @@ template <> const std::map<std::string, Precision> &names<Precision>() {
     return m;
 }
 
+template <> const std::map<std::string, LSRegressionBasis> &names<LSRegressionBasis>() {
+    static const std::map<std::string, LSRegressionBasis> m = {{"LaguerreP02", LSRegressionBasis::LaguerreP02},
+                                                               {"LaguerreP03", LSRegressionBasis::LaguerreP03},
+                                                               {"LaguerreP04", LSRegressionBasis::LaguerreP04}};
+    return m;
+}
+
 template <typename T> T string_to_enum(const std::string &input, const std::string &field) {
     const auto &m = names<T>();
     auto it = m.find(input);
@@ -67,6 +74,7 @@ template NumScheme string_to_enum<NumScheme>(const std::string &, const std::str
 template IOFormat string_to_enum<IOFormat>(const std::string &, const std::string &);
 template VerbosityLevel string_to_enum<VerbosityLevel>(const std::string &, const std::string &);
 template Precision string_to_enum<Precision>(const std::string &, const std::string &);
+template LSRegressionBasis string_to_enum<LSRegressionBasis>(const std::string &, const std::string &);
 }  // namespace Implemented
 
 namespace Config {
@@ -328,6 +336,10 @@ UInputs from_root(const yaml::Node &root) {
         if (n["time_absolute_bump_size"]) conf.mc.time_absolute_bump_size = n["time_absolute_bump_size"].as<double>();
         // This build's addition: arithmetic precision chosen at run time.
         if (n["Precision"]) conf.mc.precision = KI::string_to_enum<KI::Precision>(n["Precision"].as<std::string>(), "MC.Precision");
+        // Also this build's: the Longstaff-Schwartz basis chosen at run time.
+        if (n["LSM_Basis"])
+            conf.mc.lsm_basis = KI::string_to_enum<KI::LSRegressionBasis>(n["LSM_Basis"].as<std::string>(), "MC.LSM_Basis");
+        if (n["LSM_Weighted_Basis"]) conf.mc.lsm_weighted_basis = n["LSM_Weighted_Basis"].as<bool>();
     }
     if (const auto &n = root["Output"]; n) {
         if (n["Verbosity"])

@@ -28,6 +28,8 @@ enum class NumScheme { Euler, ImplicitMilstein, AndersonQE };
 enum class IOFormat { BIN, TXT };
 enum class VerbosityLevel : int { None = 0, Low = 1, Medium = 2, High = 3 };
 enum class Precision { FP32, FP64 };
+// Longstaff-Schwartz regression basis (ConfigFileEnums.hpp:107-112).
+enum class LSRegressionBasis { LaguerreP02, LaguerreP03, LaguerreP04 };
 
 template <typename T> const std::map<std::string, T> &names();
 template <typename T> std::string enum_to_string(T v) {
@@ -94,6 +96,10 @@ struct MCConfig {
     double spot_price_relative_bump_size = 0.001, volatility_absolute_bump_size = 0.01,
            risk_free_rate_absolute_bump_size = 0.0001, time_absolute_bump_size = 1.0 / 365.0;
     Implemented::Precision precision = Implemented::Precision::FP64;
+    // The reference fixes the LSM basis at compile time (LSMAmerican.hpp:217);
+    // here MC.LSM_Basis and MC.LSM_Weighted_Basis choose it per run.
+    Implemented::LSRegressionBasis lsm_basis = Implemented::LSRegressionBasis::LaguerreP03;
+    bool lsm_weighted_basis = false;
     void validate(const MarketConfig &m, double t_end) const;
     bool must_compute_greeks() const {
         return compute_delta_et_gamma || compute_vega_et_vomma || compute_vanna || compute_rho || compute_theta;

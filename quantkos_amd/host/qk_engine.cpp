@@ -237,7 +237,9 @@ OptionPricer::MCOpPrices price_american(const KC::UInputs &scaled, bool print) {
         check(qk_price_distribution(ctx, &c, &p, &s, &d), "qk_price_distribution");
         pr.accumulate_device_distribution(d);
     } else {
-        check(qk_price_american(ctx, &c, right, scaled.options.StrikePrice, &s, nullptr), "qk_price_american");
+        const qk_lsm_options opt{(int32_t)scaled.mc.lsm_basis, scaled.mc.lsm_weighted_basis ? 1 : 0};
+        check(qk_price_american_ex(ctx, &c, right, scaled.options.StrikePrice, &opt, &s, nullptr),
+              "qk_price_american_ex");
     }
     pr.accumulate_device_stats(s);
     pr.evaluate_option_price();

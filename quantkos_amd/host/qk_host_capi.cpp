@@ -49,6 +49,8 @@ extern "C" qk_status qk_config_load(const char *yaml_path, qk_run_spec *out, cha
         out->compute_greeks = conf.must_compute_greeks();
         if (out->compute_greeks) out->bumps = Engine::to_bumps(conf);
         out->write_paths = !conf.output.filename_paths_out.empty();
+        out->lsm = qk_lsm_options{(int32_t)conf.mc.lsm_basis, conf.mc.lsm_weighted_basis ? 1 : 0};
+        out->analyze_distribution = conf.mc.analyze_risk_neutral_payoff_distribution ? 1 : 0;
         return QK_OK;
     } catch (const std::exception &e) {
         copy_err(err, err_len, e.what());

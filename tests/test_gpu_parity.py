@@ -343,3 +343,38 @@ def test_config0_contract(engine, oracle):
     assert abs(cf - 8.92941) < 1e-4
     q = engine.price(qe.contract(scheme=qk.QE, n_paths=1_000_000, **kw), p)
     assert abs(q.price - cf) <= 3.0 * q.se, (q.price, cf, q.se)
+
+
+@pytest.mark.parametrize("basis,weighted", [(qk.LAGUERRE_P02, False), (qk.LAGUERRE_P04, False),
+                                            (qk.LAGUERRE_P03, True)])
+def test_american_other_bases_match_oracle(engine, oracle, basis, weighted):
+    """LaguerreP02 / P04 and the weighted basis: the same regression as the oracle's
+    on the same paths (FP64): continuation values over the in-the-money range
+    within 2e-3 at every date, prices within 0.05 SE.  The degree-4 Gram is
+    badly conditioned when X = S/K clusters (condition number ~1e16 at the first
+    dates of a 10 %-out-of-the-money put with v0 = 0.04, measured on the oracle),
+    where rounding alone changes the exercise policy; this contract (v0 = 0.25,
+    K = 1.5 S0, T = 2) keeps it below 4e12 at every date."""
+    nb = 3 + basis
+    c = base_contract(qk.HESTON, qk.MILSTEIN, qk.FP64, n_paths=20_000, n_steps=50, price_scale=100.0, v0=0.25,
+                      theta=0.25, sigma=0.3, T=2.0)
+    g, gco = engine.price_american(c, qk.PUT, 1.5, want_coeffs=True, basis=basis, weighted=weighted)
+    o, oco = oracle.price_american(omodel(oracle, c), c.n_paths, qk.PUT, 1.5, 100.0, want_coeffs=True,
+                                   n_basis=nb, weighted=weighted)
+    assert gco.shape == oco.shape == (c.n_steps, nb)
+    X = np.linspace(0.45, 1.0, 45)
+    Lx = np.stack([oracle.basis(x, nb, weighted) for x in X])
+    diff = np.abs(Lx @ gco[:-1].T - Lx @ oco[:-1].T).max()
+    assert diff < 2e-3, diff
+    assert abs(g.price - o.price) < 0.05 * o.se, (g.price, o.price, o.se)
+
+
+@pytest.mark.parametrize("basis,weighted", [(qk.LAGUERRE_P02, False), (qk.LAGUERRE_P04, False),
+                                            (qk.LAGUERRE_P03, True)])
+def test_american_other_bases_ikonen_toivanen(engine, basis, weighted):
+    """The other bases on the S0 = K = 10 case (0.520030) at the reference's size."""
+    c = qe.contract(model=qk.HESTON, scheme=qk.MILSTEIN, precision=qk.FP32, n_steps=1000,
+                    n_paths=500_000, seed=184467440737095, S0=1.0, v0=0.0625, r=0.1, q=0.0, T=0.25,
+                    kappa=5.0, theta=0.16, sigma=0.9, rho=0.1, price_scale=10.0)
+    s = engine.price_american(c, qk.PUT, 1.0, basis=basis, weighted=weighted)
+    assert abs(s.price - 0.520030) <= 3 * s.se + max(0.02 * 0.520030, 0.005), (s.price, s.se)

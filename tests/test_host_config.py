@@ -124,6 +124,7 @@ def mc(text, lines):
      "strictly smaller than sqrt(v0)"),
     (lambda t: mc(t, "compute_theta: true\n  time_absolute_bump_size: 3.0"), "strictly smaller than T_End"),
     (lambda t: mc(t, "Precision: FP16"), "Invalid value 'FP16'"),
+    (lambda t: mc(t, "LSM_Basis: Monomial2"), "Invalid value 'Monomial2' for MC.LSM_Basis"),
 ])
 def test_validation_errors(in_tmp, edit, msg):
     with pytest.raises(QKError) as e:
@@ -142,3 +143,13 @@ def test_cli_usage_and_fatal_error(in_tmp):
     assert r.returncode == 0 and "Usage: quantkos" in r.stdout
     r = subprocess.run([cli, str(in_tmp / "missing.yaml")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "FATAL ERROR" in r.stderr and "File not found" in r.stderr
+
+
+def test_lsm_basis_and_distribution_keys(in_tmp):
+    """This build's run-time LSM basis keys (LSRegressionBasis names, ConfigFileEnums.hpp:107-112)
+    and the reference's analyze_risk_neutral_payoff_distribution key."""
+    s = load_config(write(in_tmp, BASE))
+    assert (s.lsm.basis, s.lsm.weighted, s.analyze_distribution) == (qk.LAGUERRE_P03, 0, 0)
+    s = load_config(write(in_tmp, mc(BASE, "LSM_Basis: LaguerreP04\n  LSM_Weighted_Basis: true\n"
+                                           "  analyze_risk_neutral_payoff_distribution: true")))
+    assert (s.lsm.basis, s.lsm.weighted, s.analyze_distribution) == (qk.LAGUERRE_P04, 1, 1)

@@ -211,3 +211,41 @@ def test_greek_put_call_parity(oracle):
     exp_theta = -r * K * math.exp(-r * T)
     assert abs((gc.theta - gp.theta) - exp_theta) <= tol(exp_theta, 5.0)
     assert gc.n_pricings == 12
+
+
+def test_basis_orders_and_weight(oracle):
+    """LaguerreP02/P03/P04 = L0..L2, L0..L3, L0..L4 (LSMAmerican.hpp:172-198) against
+    scipy's Laguerre polynomials; the weighted variant is exp(-X/2) times them."""
+    from scipy.special import eval_laguerre
+    for X in (0.0, 0.37, 0.9, 1.0, 1.6, 3.2):
+        for nb in (3, 4, 5):
+            L = oracle.basis(X, nb)
+            assert np.allclose(L, [eval_laguerre(k, X) for k in range(nb)], rtol=1e-13, atol=1e-14)
+            W = oracle.basis(X, nb, weighted=True)
+            assert np.allclose(W, np.exp(-X / 2) * L, rtol=1e-15, atol=1e-16)
+
+
+@pytest.mark.parametrize("nb", [3, 5])
+def test_lsm_solve_other_orders(oracle, nb):
+    """The ridge normal equations (M + 1e-8 I) C = B for 3 and 5 basis functions
+    against numpy, and the count < nb fallback (LSMAmerican.hpp:516-613)."""
+    rng = np.random.default_rng(nb)
+    X = rng.uniform(0.5, 1.0, 400)
+    L = np.stack([oracle.basis(x, nb) for x in X])
+    Y = np.maximum(1.0 - X, 0) + 0.05 * rng.standard_normal(400)
+    M = L.T @ L
+    C, fb = oracle.lsm_solve_n(np.concatenate([[400.0], M.ravel(), L.T @ Y]), nb)
+    ref = np.linalg.solve(M + 1e-8 * np.eye(nb), L.T @ Y)
+    assert not fb and np.allclose(L @ C, L @ ref, rtol=1e-7, atol=1e-9)
+    C, fb = oracle.lsm_solve_n(np.concatenate([[nb - 1.0], np.zeros(nb * nb), [0.3] + [1.0] * (nb - 1)]), nb)
+    assert fb and C[0] == pytest.approx(0.3 / (nb - 1)) and not C[1:].any()
+
+
+@pytest.mark.parametrize("nb,weighted", [(3, False), (5, False), (4, True)])
+def test_american_other_bases_ikonen_toivanen(oracle, nb, weighted):
+    """The other regression bases on the S0 = K = 10 case of the Ikonen-Toivanen table
+    (0.520030), same tolerance rule as the reference's test."""
+    m = M(oracle, scheme=1, n_steps=250, seed=3, S0=1.0, v0=0.0625, r=0.1, q=0.0, T=0.25, kappa=5.0,
+          theta=0.16, sigma=0.9, rho=0.1)
+    s = oracle.price_american(m, 100_000, 1, 1.0, 10.0, n_basis=nb, weighted=weighted)
+    assert abs(s.price - 0.520030) <= 3 * s.se + max(0.02 * 0.520030, 0.005), (s.price, s.se)
