@@ -118,6 +118,17 @@ qk_status qk_set_schedule(qk_ctx *ctx, const qk_schedule_entry *entries, int32_t
 /* The lanes per thread a call with this key would use. */
 int32_t qk_lanes_per_thread(const qk_ctx *ctx, int32_t precision, int32_t scheme, int32_t n_lanes);
 
+/* Schedule of the Longstaff-Schwartz backward pass of qk_price_american*:
+ * QK_LSM_PER_DATE launches one kernel per exercise date; QK_LSM_COOPERATIVE
+ * runs every date in one cooperative launch (one device-wide synchronisation
+ * per date, cash flows held in LDS) when the request's cash flows fit the
+ * blocks' LDS and falls back to per-date launches otherwise; QK_LSM_AUTO is
+ * the faster of the two as measured on MI355X (per-date).  The schedules give
+ * bit-identical results.  No counterpart in the reference, whose backward
+ * pass is a host loop over dates (ForwardBackwardMC.hpp:219-242). */
+enum { QK_LSM_AUTO = 0, QK_LSM_PER_DATE = 1, QK_LSM_COOPERATIVE = 2 };
+qk_status qk_set_lsm_schedule(qk_ctx *ctx, int32_t mode);
+
 /* ---- pricing ---- */
 /* Plain single-contract launch.  Replaces MSolver<M,S,T,R>::execute_batch
  * (MSolver.hpp:195-232) + OptionPricer::accumulate_batch_payoffs /

@@ -19,6 +19,7 @@ EULER, MILSTEIN, QE = 0, 1, 2
 CALL, PUT = 0, 1
 FP32, FP64 = 0, 1
 LAGUERRE_P02, LAGUERRE_P03, LAGUERRE_P04 = 0, 1, 2
+LSM_AUTO, LSM_PER_DATE, LSM_COOPERATIVE = 0, 1, 2
 
 
 class Contract(C.Structure):
@@ -125,6 +126,7 @@ def load():
         "qk_last_timing": ([vp, P(Timing)], st),
         "qk_footprint_bytes": ([P(Contract), C.c_int32, C.c_int32, C.c_int32], C.c_size_t),
         "qk_set_schedule": ([vp, P(ScheduleEntry), C.c_int32], st),
+        "qk_set_lsm_schedule": ([vp, C.c_int32], st),
         "qk_lanes_per_thread": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int32),
         "qk_price": ([vp, P(Contract), P(Payoff), P(Stats)], st),
         "qk_price_payoffs": ([vp, P(Contract), P(Payoff), C.c_int32, P(Stats)], st),

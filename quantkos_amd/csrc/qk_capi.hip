@@ -196,7 +196,7 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
         b += (size_t)c.n_paths * (size_t)c.n_steps * rs;   // path store [date][path]
         b += (size_t)c.n_paths * rs;                         // cash flows
         int64_t chunk = 0;
-        b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 + (size_t)pump_blocks(c.n_paths, &chunk) * LSM_STRIDE * 8;
+        b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 + (size_t)2 * pump_blocks(c.n_paths, &chunk) * LSM_STRIDE * 8;
     }
     return b;
 }
@@ -243,7 +243,7 @@ extern "C" qk_status qk_ctx_create(int device, size_t mem_budget_bytes, qk_ctx *
             return QK_EDEVICE;
         }
     }
-    if (ctx->counter.reserve(ctx, 256) != QK_OK || hipMemset(ctx->counter.p, 0, 256) != hipSuccess) {
+    if (ctx->counter.reserve(ctx, 2048) != QK_OK || hipMemset(ctx->counter.p, 0, 2048) != hipSuccess) {
         delete ctx;
         return QK_EDEVICE;
     }
@@ -549,7 +549,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     const int nb = pump_blocks(n, &chunk);
     if ((st = ctx->cf.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
     if ((st = ctx->coeffs.reserve(ctx, (size_t)N * n_basis * 8)) != QK_OK) return st;
-    if ((st = ctx->bpart.reserve(ctx, (size_t)nb * LSM_STRIDE * 8)) != QK_OK) return st;
+    if ((st = ctx->bpart.reserve(ctx, (size_t)2 * nb * LSM_STRIDE * 8)) != QK_OK) return st;
     if ((st = ctx->psum.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pm2.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pitm.reserve(ctx, n_tiles * 4)) != QK_OK) return st;
@@ -582,9 +582,46 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     a.n_req = 1;
     a.block_start[0] = 0;
     a.block_start[1] = nb;
-    for (int t = N - 1; t >= 0; --t) {                   // ForwardBackwardMC.hpp:219-242
-        q.t = t;
-        QK_HIP(launch_pump<Real>(a, n_basis, weighted, s));
+    // One launch per date, or on request all dates in one cooperative launch
+    // (when the cash flows of a block's chunks fit its LDS and every block can
+    // be resident).  Both give bit-identical results; the per-date launches
+    // measure faster on MI355X, so they are the default.
+    const int coop_blocks = (nb + LSM_COOP_SUBS - 1) / LSM_COOP_SUBS;
+    const size_t coop_lds = (size_t)LSM_COOP_SUBS * chunk * sizeof(Real);
+    bool coop = false;
+    if (ctx->lsm_mode == QK_LSM_COOPERATIVE) {
+        int max_lds = 0, n_cu = 0, coop_ok = 0;
+        (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+        (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+        (void)hipDeviceGetAttribute(&coop_ok, hipDeviceAttributeCooperativeLaunch, ctx->device);
+        coop = coop_ok && coop_lds + 8192 <= (size_t)max_lds &&
+               (int64_t)lsm_coop_occupancy<Real>(n_basis, weighted, coop_lds) * n_cu >= coop_blocks;
+    }
+    if (coop) {
+        LsmCoopArgs<Real> ca{};
+        ca.store = q.store;
+        ca.cf = q.cf;
+        ca.bpart = q.bpart;
+        ca.coeffs = q.coeffs;
+        ca.psum = q.psum;
+        ca.pm2 = q.pm2;
+        ca.pitm = q.pitm;
+        ca.sync = (unsigned int *)ctx->counter.p;
+        ca.n_paths = n;
+        ca.chunk = chunk;
+        ca.n_steps = N;
+        ca.right = right;
+        ca.nb = nb;
+        ca.K = q.K;
+        ca.disc = q.disc;
+        ca.itm_eps = q.itm_eps;
+        QK_HIP(hipMemsetAsync(ctx->counter.p, 0, 2048, s));
+        QK_HIP(launch_lsm_coop<Real>(ca, n_basis, weighted, coop_blocks, coop_lds, s));
+    } else {
+        for (int t = N - 1; t >= 0; --t) {               // ForwardBackwardMC.hpp:219-242
+            q.t = t;
+            QK_HIP(launch_pump<Real>(a, n_basis, weighted, s));
+        }
     }
     QK_HIP(hipEventRecord(ctx->ev[3], s));
     QK_HIP(launch_merge((const double *)ctx->psum.p, (const double *)ctx->pm2.p, (const uint32_t *)ctx->pitm.p,
@@ -594,7 +631,11 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, s));
     if (coeffs_out)
         QK_HIP(hipMemcpyAsync(coeffs_out, ctx->coeffs.p, (size_t)N * n_basis * 8, hipMemcpyDeviceToHost, s));
+    unsigned int sync_err = 0;
+    if (coop) QK_HIP(hipMemcpyAsync(&sync_err, (char *)ctx->counter.p + 4 * LSM_COOP_SYNC_ERR_WORD, 4,
+                                    hipMemcpyDeviceToHost, s));
     QK_HIP(hipStreamSynchronize(s));
+    if (sync_err) return fail(ctx, QK_EDEVICE, "Longstaff-Schwartz grid barrier timed out");
     // Immediate exercise check (LSMAmerican.hpp:409-418): if today's intrinsic
     // beats the mean discounted cash flow, every path exercises at t0.
     const Real S0 = (Real)c.S0, Kr = (Real)K;
@@ -754,6 +795,13 @@ extern "C" qk_status qk_price_distribution(qk_ctx *ctx, const qk_contract *c, co
     QK_HIP(hipSetDevice(ctx->device));
     return c->precision == QK_FP32 ? run_distribution<float>(ctx, *c, *p, out, dist)
                                    : run_distribution<double>(ctx, *c, *p, out, dist);
+}
+
+extern "C" qk_status qk_set_lsm_schedule(qk_ctx *ctx, int32_t mode) {
+    if (!ctx) return QK_EINVAL;
+    if (mode < QK_LSM_AUTO || mode > QK_LSM_COOPERATIVE) return fail(ctx, QK_EINVAL, "lsm schedule out of range");
+    ctx->lsm_mode = mode;
+    return QK_OK;
 }
 
 extern "C" qk_status qk_price_american_ex(qk_ctx *ctx, const qk_contract *c, int32_t right, double K,

@@ -41,5 +41,6 @@ struct qk_ctx {
     qk::DevBuf lanes, pays, psum, pm2, pitm, disc, scale, stats, greeks, store, cf, coeffs, bpart, counter, sums, mscratch;
     qk::DevBuf payv, hist;   // per-path payoffs and selection histograms of qk_price_distribution
     std::vector<qk_schedule_entry> schedule;   // lanes per thread by (precision, scheme, n_lanes)
+    int32_t lsm_mode = QK_LSM_AUTO;             // backward-pass schedule of qk_price_american*
     ~qk_ctx();
 };

@@ -68,6 +68,10 @@ template <> struct Math<float> {
     __device__ static __forceinline__ float neg2log(float u) {
         return __builtin_amdgcn_logf(u) * (-2.0f * 0.6931471805599453f);
     }
+    // ln(x) / 2, equal to 0.5f * log(x) for the same reason.
+    __device__ static __forceinline__ float half_log(float x) {
+        return __builtin_amdgcn_logf(x) * (0.5f * 0.6931471805599453f);
+    }
     // sin/cos of 2*pi*u for u in (0, 1).
     __device__ static __forceinline__ void sincos2pi(float u, float &s, float &c) {
         s = __builtin_amdgcn_sinf(u);
@@ -90,6 +94,7 @@ template <> struct Math<double> {
     __device__ static __forceinline__ double track_max(double a, double b) { return ::fmax(a, b); }
     __device__ static __forceinline__ double track_min(double a, double b) { return ::fmin(a, b); }
     __device__ static __forceinline__ double neg2log(double u) { return -2.0 * ::log(u); }
+    __device__ static __forceinline__ double half_log(double x) { return 0.5 * ::log(x); }
     __device__ static __forceinline__ void sincos2pi(double u, double &s, double &c) {
         ::sincospi(2.0 * u, &s, &c);
     }
@@ -148,7 +153,7 @@ __device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real 
                     const Real safe_A = M::fmin(c.A, Real(0.5) * (Real(1) + b2) * rm - c.eps);
                     const Real tAa = Real(1) - Real(2) * safe_A * a;
                     // log M = A b^2 a / (1 - 2 A a) - log(1 - 2 A a) / 2
-                    logM = M::div(safe_A * b2 * a, tAa) - Real(0.5) * M::log(tAa);
+                    logM = M::div(safe_A * b2 * a, tAa) - M::half_log(tAa);
                 } else {
                     const Real p = M::div(psi - Real(1), psi + Real(1));
                     const Real beta = (Real(1) - p) * rm;

@@ -9,11 +9,13 @@
 //  k_merge      fixed-order Chan merge of tile moments + FP64 finalisation
 //               (OptionPricer.hpp:152-172, :204-249).
 //  k_greeks     FP64 central differences of lane prices (MCEngine.hpp:586-730).
-//  k_lsm_pump   one Longstaff-Schwartz date for every listed request: FP64
+//  k_lsm_step   one Longstaff-Schwartz date for every listed request: FP64
 //               Cholesky solve of the date's normal equations, cash-flow update,
 //               Laguerre Gram / right-hand side of the previous date, and at
 //               date 0 the final discount and tile moments (LSMAmerican.hpp:
 //               310-341, :393-419, :433-473, :516-613).
+//  k_lsm_coop   the same dates of one request in one cooperative launch.
+//  k_select_hist  order statistics of per-path payoffs (distribution).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,13 +48,18 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
     const uint64_t gpath = (uint64_t)(path0 + path);     // counter of the random stream
     const uint32_t p0 = (uint32_t)gpath, p1 = (uint32_t)(gpath >> 32);
 
+    // Lane constants held in registers for the whole path loop (scalar
+    // registers: the lane block is uniform per wavefront).
+    LaneC<Real> cl[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) cl[l] = lc[l];
     Real S[L], v[L], mx[L], mn[L];
     double sm[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-        S[l] = PATHDEP ? lc[l].S0 : Real(0);     // log-increment accumulator when !PATHDEP
-        v[l] = lc[l].v0;
-        mx[l] = mn[l] = lc[l].S0;
+        S[l] = PATHDEP ? cl[l].S0 : Real(0);     // log-increment accumulator when !PATHDEP
+        v[l] = cl[l].v0;
+        mx[l] = mn[l] = cl[l].S0;
         sm[l] = 0.0;
     }
 
@@ -65,10 +72,10 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
         const Real z1 = R * cs, z2 = R * sn;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            Real inc = heston_step<Real, SCHEME>(lc[l], z1, z2, w.w2, v[l]);
+            Real inc = heston_step<Real, SCHEME>(cl[l], z1, z2, w.w2, v[l]);
             if constexpr (MODEL == MODEL_BATES) {
                 const Real u3 = M::uniform(w.w3);
-                if (u3 >= lc[l].eml_dt) inc += bates_jump_slow<Real>(lc[l], u3, p0, p1, (uint32_t)t, key0, key1);
+                if (u3 >= cl[l].eml_dt) inc += bates_jump_slow<Real>(cl[l], u3, p0, p1, (uint32_t)t, key0, key1);
             }
             if constexpr (PATHDEP) {
                 S[l] *= M::exp(inc);
@@ -89,7 +96,7 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
 #pragma unroll
     for (int l = 0; l < L; ++l) {
         if (l >= n_valid) break;
-        const Real ST = PATHDEP ? S[l] : lc[l].S0 * M::exp(S[l]);
+        const Real ST = PATHDEP ? S[l] : cl[l].S0 * M::exp(S[l]);
         for (int j = 0; j < n_pay; ++j) {
             const Real x = active ? payoff_eval<Real>(pays[j], ST, sm[l], mx[l], mn[l], n_steps) : Real(0);
             if (pay_out != nullptr && l == 0 && j == 0 && active) pay_out[path] = x;
@@ -297,146 +304,334 @@ __device__ void lsm_solve(const double *s, double C[NB]) {
     }
 }
 
-// Regression step of one Longstaff-Schwartz date for every request listed in
-// the arguments, one block per request: sums the block partials of the
-// in-the-money Gram matrix / right-hand side of date t (written by the pump
-// step of date t+1) in block order and solves the NB x NB normal equations in
-// FP64 into coeffs[t].  Requests at their last date have nothing to regress.
-template <typename Real, int NB>
-__global__ __launch_bounds__(256) void k_lsm_coef(PumpArgs<Real> a) {
-    constexpr int NS = LsmSums<NB>::NS;
-    const PumpReq<Real> &q = a.r[blockIdx.x];
-    const int t = q.t;
-    if (t >= q.n_steps - 1) return;
-    __shared__ double red[NS][256];
-    double acc[NS];
+// Sums the block partials [nb][STRIDE] of one date's regression in a fixed
+// order and solves the NB x NB normal equations in FP64 (256 threads; the
+// partials are read as one coalesced stream: STRIDE divides 256, so thread i
+// always meets component i % STRIDE).  Every thread of the block must call it
+// (it holds block barriers); only `active` threads read, and thread 0 of an
+// active group leaves the coefficients in C.
+template <int NB>
+__device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpart, int nb, double *red, bool active,
+                                                 double C[NB]) {
+    using S = LsmSums<NB>;
+    constexpr int NS = S::NS, ST = S::STRIDE;
+    const int lt = threadIdx.x & 255;
+    constexpr int UR = 16;  // loads in flight per thread; partial sums combined in a fixed order
+    double acc[UR];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
-    for (int j = threadIdx.x; j < q.nb; j += 256) {
+    for (int u = 0; u < UR; ++u) acc[u] = 0.0;
+    const int64_t total = active ? (int64_t)nb * ST : 0;
+    for (int64_t e0 = lt; e0 < total; e0 += 256 * UR) {
 #pragma unroll
-        for (int k = 0; k < NS; ++k) acc[k] += q.bpart[(int64_t)j * LSM_STRIDE + k];
-    }
-#pragma unroll
-    for (int k = 0; k < NS; ++k) red[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-#pragma unroll
-            for (int k = 0; k < NS; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        for (int u = 0; u < UR; ++u) {
+            const int64_t e = e0 + (int64_t)u * 256;
+            if (e < total) acc[u] += bpart[e];
         }
+    }
+    double tot = 0.0;
+#pragma unroll
+    for (int u = 0; u < UR; ++u) tot += acc[u];
+    if (active) red[lt] = tot;
+    __syncthreads();
+    for (int s = 128; s >= ST; s >>= 1) {
+        if (active && lt < s) red[lt] += red[lt + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double sm[NS], C[NB];
-        for (int k = 0; k < NS; ++k) sm[k] = red[k][0];
+    if (active && lt == 0) {
+        double sm[NS];
+        for (int k = 0; k < NS; ++k) sm[k] = red[k];
         lsm_solve<NB>(sm, C);
-        for (int k = 0; k < NB; ++k) q.coeffs[(int64_t)t * NB + k] = C[k];
     }
 }
 
-// Path step of one Longstaff-Schwartz date for every request listed in the
-// arguments ("pump" step).  Request q's blocks each cover `chunk` consecutive
-// paths.  At date t a block (1) updates the cash flows at date t with the
-// coefficients coeffs[t] (initialises them at the last date), (2) accumulates
-// the in-the-money Gram sums of date t-1 in FP64 into its partial, and at date
-// 0 applies the final one-step discount and writes the per-tile moments of the
-// cash flows.
+// One Longstaff-Schwartz date t over the paths [p0, p1) by a group of 256
+// threads (threadIdx.x & 255 is the thread's place in the group): (1) updates
+// the cash flows at date t with the coefficients C of date t (initialises them
+// at the last date); (2) for t >= 1 accumulates the in-the-money Gram sums of
+// date t-1 in FP64 and stores the group's sums in part[0 .. STRIDE) (nothing
+// if part is null); (3) at date 0 applies the final one-step discount and
+// writes the per-tile moments of the cash flows.  cfp[i - p0] is the cash flow
+// of path i (device or LDS memory).  Holds a block barrier when t >= 1: every
+// thread of the block must call it for the same t.
 template <typename Real, int NB, bool W>
-__global__ __launch_bounds__(256) void k_lsm_pump(PumpArgs<Real> a) {
+__device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, Real *cfp, int64_t n, int64_t p0,
+                                               int64_t p1, int t, int N, int right, Real K, Real disc, Real itm_eps,
+                                               const double C[NB], double *part, double *psum, double *pm2,
+                                               uint32_t *pitm, double (*wpart)[LsmSums<NB>::NS]) {
     using S = LsmSums<NB>;
     constexpr int NS = S::NS;
-    int r = 0;
-    while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
-    const PumpReq<Real> &q = a.r[r];
-    const int b = (int)blockIdx.x - a.block_start[r];
-    const int t = q.t, N = q.n_steps;
-    const int64_t n = q.n_paths;
-    double C[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? q.coeffs[(int64_t)t * NB + k] : 0.0;
+    const int lt = threadIdx.x & 255, lid = lt & 63, wv = lt >> 6;
     double g[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) g[k] = 0.0;
-    const int lid = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t p0 = (int64_t)b * q.chunk;
-    const int64_t p1 = p0 + q.chunk < n ? p0 + q.chunk : n;
-    for (int64_t base = p0; base < p1; base += 256) {
-        const int64_t i = base + threadIdx.x;
-        const bool act = i < p1;
-        Real c = Real(0);
-        if (act) {
-            const Real S = q.store[(int64_t)t * n + i];
-            const Real intrinsic = vanilla<Real>(q.right, S, q.K);
-            if (t == N - 1) {
-                c = intrinsic;
-            } else {
-                const Real dcf = q.cf[i] * q.disc;
-                c = dcf;
-                if (intrinsic > Real(0)) {
-                    Real L[NB];
-                    lsm_basis<Real, NB, W>(Math<Real>::div(S, q.K), L);
-                    double cont = 0.0;
+    constexpr int U = 8;    // 256-path rows per pass; their loads are issued together
+    for (int64_t base = p0; base < p1; base += 256 * U) {
+        Real Sv[U], S1v[U], cfv[U];
+        bool act[U];
 #pragma unroll
-                    for (int k = 0; k < NB; ++k) cont += C[k] * (double)L[k];
-                    if ((double)intrinsic > cont) c = intrinsic;
-                }
-            }
-            if (t >= 1) {
-                const Real S1 = q.store[(int64_t)(t - 1) * n + i];
-                if (vanilla<Real>(q.right, S1, q.K) > Real(0)) {
-                    Real Lr[NB];
-                    lsm_basis<Real, NB, W>(Math<Real>::div(S1, q.K), Lr);
-                    const double Y = (double)(c * q.disc);
-                    double L[NB];
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = base + u * 256 + lt;
+            act[u] = i < p1;
+            Sv[u] = act[u] ? store[(int64_t)t * n + i] : Real(0);
+            S1v[u] = (act[u] && t >= 1) ? store[(int64_t)(t - 1) * n + i] : Real(0);
+            cfv[u] = (act[u] && t < N - 1) ? cfp[i - p0] : Real(0);
+        }
 #pragma unroll
-                    for (int k = 0; k < NB; ++k) L[k] = (double)Lr[k];
-                    g[0] += 1.0;
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = base + u * 256 + lt;
+            Real c = Real(0);
+            if (act[u]) {
+                const Real Sx = Sv[u];
+                const Real intrinsic = vanilla<Real>(right, Sx, K);
+                if (t == N - 1) {
+                    c = intrinsic;
+                } else {
+                    const Real dcf = cfv[u] * disc;
+                    c = dcf;
+                    if (intrinsic > Real(0)) {
+                        Real L[NB];
+                        lsm_basis<Real, NB, W>(Math<Real>::div(Sx, K), L);
+                        double cont = 0.0;
 #pragma unroll
-                    for (int u = 0; u < NB; ++u) {
-#pragma unroll
-                        for (int v = u; v < NB; ++v) g[S::m(u, v)] += L[u] * L[v];
-                        g[S::b(u)] += L[u] * Y;
+                        for (int k = 0; k < NB; ++k) cont += C[k] * (double)L[k];
+                        if ((double)intrinsic > cont) c = intrinsic;
                     }
                 }
-            } else {
-                c = c * q.disc;   // LSMAmerican.hpp:393-405
+                if (t >= 1) {
+                    const Real S1 = S1v[u];
+                    if (vanilla<Real>(right, S1, K) > Real(0)) {
+                        Real Lr[NB];
+                        lsm_basis<Real, NB, W>(Math<Real>::div(S1, K), Lr);
+                        const double Y = (double)(c * disc);
+                        double L[NB];
+#pragma unroll
+                        for (int k = 0; k < NB; ++k) L[k] = (double)Lr[k];
+                        g[0] += 1.0;
+#pragma unroll
+                        for (int a2 = 0; a2 < NB; ++a2) {
+#pragma unroll
+                            for (int b2 = a2; b2 < NB; ++b2) g[S::m(a2, b2)] += L[a2] * L[b2];
+                            g[S::b(a2)] += L[a2] * Y;
+                        }
+                    }
+                } else {
+                    c = c * disc;   // LSMAmerican.hpp:393-405
+                }
+                cfp[i - p0] = c;
             }
-            q.cf[i] = c;
-        }
-        if (t == 0) {
-            const unsigned long long am = __ballot(act);
-            if (am) {
-                const double n_act = (double)__popcll(am);
-                const double xd = (double)c;
-                const double s = wave_sum(xd);
-                const double mean = s / n_act;
-                const double d = act ? xd - mean : 0.0;
-                const double m2 = wave_sum(d * d);
-                const unsigned long long im = __ballot(act && c > q.itm_eps);
-                if (lid == 0) {
-                    const int64_t tile = base / TILE + wv;
-                    q.psum[tile] = s;
-                    q.pm2[tile] = m2;
-                    q.pitm[tile] = (uint32_t)__popcll(im);
+            if (t == 0) {
+                const unsigned long long am = __ballot(act[u]);
+                if (am) {
+                    const double n_act = (double)__popcll(am);
+                    const double xd = (double)c;
+                    const double sx = wave_sum(xd);
+                    const double mean = sx / n_act;
+                    const double d = act[u] ? xd - mean : 0.0;
+                    const double m2 = wave_sum(d * d);
+                    const unsigned long long im = __ballot(act[u] && c > itm_eps);
+                    if (lid == 0) {
+                        const int64_t tile = (base + u * 256) / TILE + wv;
+                        psum[tile] = sx;
+                        pm2[tile] = m2;
+                        pitm[tile] = (uint32_t)__popcll(im);
+                    }
                 }
             }
         }
     }
     if (t >= 1) {
-        __shared__ double wpart[WAVES_PER_BLOCK][NS];
+        // Wavefront sums of the NS components by a reduce-scatter butterfly
+        // over KP = 16 or 32 padded components: at lane offsets 32, 16, ...
+        // each lane keeps half of its values and sends the other half, so KP
+        // components cost KP - 1 shuffles (plus 6 - log2 KP full steps)
+        // instead of 6 per component.  Lane l ends with component
+        // l >> (6 - log2 KP) summed over the wavefront.
+        constexpr int KP = NS <= 16 ? 16 : 32;
+        constexpr int LOGK = KP == 16 ? 4 : 5;
+        double v[KP];
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const double s = wave_sum(g[k]);
-            if (lid == 0) wpart[wv][k] = s;
+        for (int k = 0; k < KP; ++k) v[k] = k < NS ? g[k] : 0.0;
+#pragma unroll
+        for (int o = 32, h = KP / 2; h >= 1; o >>= 1, h >>= 1) {
+            const bool hi = (lid & o) != 0;
+#pragma unroll
+            for (int k = 0; k < h; ++k) {
+                const double keep = hi ? v[k + h] : v[k];
+                const double send = hi ? v[k] : v[k + h];
+                v[k] = keep + __shfl_xor(send, o, 64);
+            }
         }
+        double sx = v[0];
+#pragma unroll
+        for (int o = (64 >> LOGK) / 2; o >= 1; o >>= 1) sx += __shfl_xor(sx, o, 64);
+        const int comp = lid >> (6 - LOGK);
+        if ((lid & ((1 << (6 - LOGK)) - 1)) == 0 && comp < NS) wpart[wv][comp] = sx;
         __syncthreads();
-        if ((int)threadIdx.x < NS) {
-            double s = 0.0;
-            for (int w = 0; w < WAVES_PER_BLOCK; ++w) s += wpart[w][threadIdx.x];
-            q.bpart[(int64_t)b * LSM_STRIDE + threadIdx.x] = s;
+        if (part != nullptr && lt < S::STRIDE) {          // the padding components are written as zero
+            double sx = 0.0;
+            if (lt < NS)
+                for (int w = 0; w < WAVES_PER_BLOCK; ++w) sx += wpart[w][lt];
+            part[lt] = sx;
         }
     }
 }
+
+// One Longstaff-Schwartz date for every request listed in the arguments, one
+// launch per date: every block of request q first sums the Gram partials of
+// date t (written by the previous launch into buffer t & 1) in block order and
+// solves the normal equations itself (block 0 also stores the coefficients),
+// then runs the date over its `chunk` of paths and writes its partial of date
+// t-1 into buffer (t-1) & 1.  The redundant solves read nb partials per block;
+// nb is kept at most PUMP_MAX_BLOCKS for that reason.
+template <typename Real, int NB, bool W>
+__global__ __launch_bounds__(256) void k_lsm_step(PumpArgs<Real> a) {
+    using S = LsmSums<NB>;
+    int r = 0;
+    while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
+    const PumpReq<Real> &q = a.r[r];
+    const int b = (int)blockIdx.x - a.block_start[r];
+    const int t = q.t, N = q.n_steps;
+    __shared__ double red[256];
+    __shared__ double Cs[NB];
+    __shared__ double wpart[WAVES_PER_BLOCK][S::NS];
+    if (t < N - 1) {
+        double Cn[NB];
+        lsm_reduce_solve<NB>(q.bpart + (int64_t)(t & 1) * q.nb * S::STRIDE, q.nb, red, true, Cn);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < NB; ++k) {
+                Cs[k] = Cn[k];
+                if (b == 0) q.coeffs[(int64_t)t * NB + k] = Cn[k];
+            }
+    }
+    __syncthreads();
+    double C[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? Cs[k] : 0.0;
+    const int64_t p0 = (int64_t)b * q.chunk;
+    const int64_t p1 = p0 + q.chunk < q.n_paths ? p0 + q.chunk : q.n_paths;
+    double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * S::STRIDE + (int64_t)b * S::STRIDE : nullptr;
+    lsm_chunk_date<Real, NB, W>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
+                                part, q.psum, q.pm2, q.pitm, wpart);
+}
+
+// Synchronisation of the cooperative backward pass (sync words, agent scope):
+// arrivals are counted per group of blocks (block b in group b % G_R, one
+// counter per 64-byte line) and groups on a top counter, so no address takes
+// more than max(G_R, G / G_R) increments per date.  The block whose arrival
+// completes the count sums the partials and solves; it publishes the
+// coefficients and a generation number, which every other block polls.  The
+// polls are bounded: on timeout the error word is raised and each block
+// leaves at its next wait, so the kernel always ends.
+constexpr int COOP_GROUPS = 8;
+constexpr int SYNC_GRP = 0, SYNC_TOP = 16 * COOP_GROUPS, SYNC_GEN = SYNC_TOP + 16, SYNC_ERR = SYNC_GEN + 16;
+constexpr int SYNC_COEF_BYTES = 1024;   // published coefficients: double [2][8] at this byte offset
+static_assert(SYNC_ERR == LSM_COOP_SYNC_ERR_WORD, "error word index shared with the host");
+
+// Returns true in every thread of the block that completed date k's count.
+__device__ __forceinline__ bool coop_arrive(unsigned int *sync, unsigned int k) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int G = gridDim.x, GR = G < (unsigned)COOP_GROUPS ? G : (unsigned)COOP_GROUPS;
+        const unsigned int g = blockIdx.x % GR, in_group = (G - g + GR - 1) / GR;
+        int is_last = 0;
+        const unsigned int old =
+            __hip_atomic_fetch_add(&sync[SYNC_GRP + 16 * g], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == k * in_group) {
+            const unsigned int top =
+                __hip_atomic_fetch_add(&sync[SYNC_TOP], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            is_last = top + 1 == k * GR;
+        }
+        last = is_last;
+    }
+    __syncthreads();
+    return last != 0;
+}
+
+// Waits for generation k; false (in every thread) on timeout or error.
+__device__ __forceinline__ bool coop_wait(unsigned int *sync, unsigned int k) {
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        int good = 1;
+        unsigned int spins = 0;
+        // Relaxed polls (an acquire per poll would invalidate the cache each
+        // time); one acquire fence once the generation is seen.
+        while (__hip_atomic_load(&sync[SYNC_GEN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
+            if (__hip_atomic_load(&sync[SYNC_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                good = 0;
+                break;
+            }
+            if (++spins > LSM_COOP_SPIN_LIMIT) {
+                __hip_atomic_store(&sync[SYNC_ERR], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// The whole backward pass of one request in one cooperative launch: block g
+// holds the chunks of virtual blocks LSM_COOP_SUBS*g + s (s = its 256-thread
+// groups), the same partition, arithmetic and reduction order as the
+// launch-per-date k_lsm_step (so the results are bit-identical), with the
+// cash flows in LDS.  Per date: partials, arrival, and either the solve (the
+// block that completes the count) or the wait for its published
+// coefficients.  Partials and published coefficients alternate between two
+// buffers by date parity.
+template <typename Real, int NB, bool W>
+__global__ __launch_bounds__(256 * LSM_COOP_SUBS) void k_lsm_coop(LsmCoopArgs<Real> a) {
+    using S = LsmSums<NB>;
+    extern __shared__ unsigned char lds_raw[];
+    Real *cf_lds = reinterpret_cast<Real *>(lds_raw);
+    __shared__ double wpart[LSM_COOP_SUBS][WAVES_PER_BLOCK][S::NS];
+    __shared__ double red[256];
+    __shared__ double Cs[NB];
+    double *cpub = reinterpret_cast<double *>(reinterpret_cast<unsigned char *>(a.sync) + SYNC_COEF_BYTES);
+    const int sub = threadIdx.x >> 8;
+    const int vb = (int)blockIdx.x * LSM_COOP_SUBS + sub;
+    const bool live = vb < a.nb;
+    const int64_t p0 = live ? (int64_t)vb * a.chunk : 0;
+    const int64_t p1 = live ? (p0 + a.chunk < a.n_paths ? p0 + a.chunk : a.n_paths) : 0;
+    Real *cfp = cf_lds + (int64_t)sub * a.chunk;
+    const int N = a.n_steps;
+    unsigned int k_bar = 0;
+    for (int t = N - 1; t >= 0; --t) {
+        double C[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? Cs[k] : 0.0;
+        const int par = (t - 1) & 1;
+        double *part = (t >= 1 && live) ? a.bpart + (int64_t)par * a.nb * S::STRIDE + (int64_t)vb * S::STRIDE : nullptr;
+        lsm_chunk_date<Real, NB, W>(a.store, cfp, a.n_paths, p0, p1, t, N, a.right, a.K, a.disc, a.itm_eps, C, part,
+                                    a.psum, a.pm2, a.pitm, wpart[sub]);
+        if (t >= 1) {
+            ++k_bar;
+            if (coop_arrive(a.sync, k_bar)) {
+                double Cn[NB];
+                lsm_reduce_solve<NB>(a.bpart + (int64_t)par * a.nb * S::STRIDE, a.nb, red, sub == 0, Cn);
+                if (threadIdx.x == 0) {
+                    for (int k = 0; k < NB; ++k) {
+                        Cs[k] = Cn[k];
+                        cpub[par * 8 + k] = Cn[k];
+                        a.coeffs[(int64_t)(t - 1) * NB + k] = Cn[k];
+                    }
+                    __hip_atomic_store(&a.sync[SYNC_GEN], k_bar, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            } else {
+                if (!coop_wait(a.sync, k_bar)) return;
+                if (threadIdx.x < NB) Cs[threadIdx.x] = cpub[par * 8 + threadIdx.x];
+            }
+            __syncthreads();
+        }
+    }
+    // Final cash flows to device memory (payoff distribution, path-set users).
+    for (int64_t i = p0 + (threadIdx.x & 255); i < p1; i += 256) a.cf[i] = cfp[i - p0];
+}
+
 
 // ---------------------------------------------------------------------------
 // Serving: one batch of requests of uneven sizes in one launch.  Work items
@@ -649,9 +844,55 @@ int pump_blocks(int64_t n_paths, int64_t *chunk) {
 
 template <typename Real, int NB, bool W>
 static void launch_pump_nb(const PumpArgs<Real> &a, hipStream_t s) {
-    hipLaunchKernelGGL((k_lsm_coef<Real, NB>), dim3(a.n_req), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((k_lsm_pump<Real, NB, W>), dim3(a.block_start[a.n_req]), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_lsm_step<Real, NB, W>), dim3(a.block_start[a.n_req]), dim3(256), 0, s, a);
 }
+
+template <typename Real, int NB, bool W>
+static hipError_t launch_coop_nb(const LsmCoopArgs<Real> &a, int blocks, size_t lds, hipStream_t s) {
+    void *args[] = {const_cast<LsmCoopArgs<Real> *>(&a)};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_lsm_coop<Real, NB, W>), dim3(blocks),
+                                      dim3(256 * LSM_COOP_SUBS), args, (unsigned int)lds, s);
+}
+
+template <typename Real, int NB, bool W>
+static int coop_occupancy(size_t lds) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_lsm_coop<Real, NB, W>),
+                                                     256 * LSM_COOP_SUBS, lds) != hipSuccess)
+        return 0;
+    return per_cu;
+}
+
+template <typename Real>
+int lsm_coop_occupancy(int n_basis, bool weighted, size_t lds) {
+    switch (n_basis * 2 + (weighted ? 1 : 0)) {
+    case 6: return coop_occupancy<Real, 3, false>(lds);
+    case 7: return coop_occupancy<Real, 3, true>(lds);
+    case 8: return coop_occupancy<Real, 4, false>(lds);
+    case 9: return coop_occupancy<Real, 4, true>(lds);
+    case 10: return coop_occupancy<Real, 5, false>(lds);
+    case 11: return coop_occupancy<Real, 5, true>(lds);
+    default: return 0;
+    }
+}
+
+template <typename Real>
+hipError_t launch_lsm_coop(const LsmCoopArgs<Real> &a, int n_basis, bool weighted, int blocks, size_t lds,
+                           hipStream_t s) {
+    switch (n_basis * 2 + (weighted ? 1 : 0)) {
+    case 6: return launch_coop_nb<Real, 3, false>(a, blocks, lds, s);
+    case 7: return launch_coop_nb<Real, 3, true>(a, blocks, lds, s);
+    case 8: return launch_coop_nb<Real, 4, false>(a, blocks, lds, s);
+    case 9: return launch_coop_nb<Real, 4, true>(a, blocks, lds, s);
+    case 10: return launch_coop_nb<Real, 5, false>(a, blocks, lds, s);
+    case 11: return launch_coop_nb<Real, 5, true>(a, blocks, lds, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+template int lsm_coop_occupancy<float>(int, bool, size_t);
+template int lsm_coop_occupancy<double>(int, bool, size_t);
+template hipError_t launch_lsm_coop<float>(const LsmCoopArgs<float> &, int, bool, int, size_t, hipStream_t);
+template hipError_t launch_lsm_coop<double>(const LsmCoopArgs<double> &, int, bool, int, size_t, hipStream_t);
 
 template <typename Real>
 hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s) {

@@ -20,20 +20,21 @@ template <int NB>
 struct LsmSums {
     static constexpr int NM = NB * (NB + 1) / 2;
     static constexpr int NS = 1 + NM + NB;
+    static constexpr int STRIDE = NS <= 16 ? 16 : 32;   // doubles per block partial; divides 256
     __host__ __device__ static constexpr int m(int i, int j) { return 1 + i * NB - i * (i - 1) / 2 + (j - i); }
     __host__ __device__ static constexpr int b(int k) { return 1 + NM + k; }
 };
 constexpr int LSM_MAX_BASIS = 5;
-constexpr int LSM_STRIDE = 24;          // doubles per block partial (>= LsmSums<5>::NS = 21)
+constexpr int LSM_STRIDE = 32;          // block-partial allocation in doubles (>= every LsmSums<NB>::STRIDE)
 
-// One request of a Longstaff-Schwartz step (k_lsm_coef + k_lsm_pump).
+// One request of a Longstaff-Schwartz date step (k_lsm_step).
 constexpr int PUMP_MAX_REQ = 24;        // requests per pump launch (kernel-argument size)
-constexpr int PUMP_MAX_BLOCKS = 1024;   // blocks per request per date
+constexpr int PUMP_MAX_BLOCKS = 256;    // blocks per request per date (each re-reads all partials)
 template <typename Real>
 struct PumpReq {
     const Real *store;    // [n_steps][n_paths]
     Real *cf;             // [n_paths]
-    double *bpart;        // [nb][LSM_STRIDE] Gram partials of the date below t
+    double *bpart;        // [2][nb][LsmSums<NB>::STRIDE] Gram partials, date d in buffer d & 1
     double *coeffs;       // [n_steps][n_basis]
     double *psum, *pm2;   // tile moments of the final cash flows
     uint32_t *pitm;
@@ -57,6 +58,30 @@ struct SelectArgs {
 template <typename Real>
 hipError_t launch_select_hist(const Real *vals, int64_t n, const SelectArgs &a, uint32_t *hist, int blocks,
                               hipStream_t s);
+
+// One request's whole backward pass in one cooperative launch (k_lsm_coop):
+// LSM_COOP_SUBS groups of 256 threads per block, one pump "virtual block" each.
+constexpr int LSM_COOP_SUBS = 2;
+constexpr unsigned int LSM_COOP_SPIN_LIMIT = 1u << 22;   // polls per wait before giving up
+constexpr int LSM_COOP_SYNC_ERR_WORD = 16 * 8 + 32;       // index of the error word in LsmCoopArgs::sync
+template <typename Real>
+struct LsmCoopArgs {
+    const Real *store;    // [n_steps][n_paths]
+    Real *cf;             // [n_paths], final cash flows
+    double *bpart;        // [2][nb][STRIDE], by date parity
+    double *coeffs;       // [n_steps][n_basis]
+    double *psum, *pm2;
+    uint32_t *pitm;
+    unsigned int *sync;   // 2 KiB of counters, flags and published coefficients; zero before the launch
+    int64_t n_paths, chunk;
+    int32_t n_steps, right, nb, pad;
+    Real K, disc, itm_eps, pad2;
+};
+template <typename Real>
+int lsm_coop_occupancy(int n_basis, bool weighted, size_t lds);
+template <typename Real>
+hipError_t launch_lsm_coop(const LsmCoopArgs<Real> &a, int n_basis, bool weighted, int blocks, size_t lds,
+                           hipStream_t s);
 
 // Blocks and paths per block of one request's pump work.
 int pump_blocks(int64_t n_paths, int64_t *chunk);

@@ -174,7 +174,7 @@ Layout layout_of(const qk_contract &c, bool american, int64_t n_tiles, int64_t n
         L.store = take((size_t)c.n_paths * c.n_steps * rs);
         L.cf = take((size_t)c.n_paths * rs);
         L.coeffs = take((size_t)c.n_steps * 4 * 8);
-        L.bpart = take((size_t)nb * LSM_STRIDE * 8);
+        L.bpart = take((size_t)2 * nb * LSM_STRIDE * 8);
         L.mreq = take(sizeof(MergeReqD));
         L.creq = take(n_chunks * 4);
         L.stats = take(sizeof(qk_stats));

@@ -99,6 +99,10 @@ class Engine:
         arr = (ScheduleEntry * max(1, len(entries)))(*[ScheduleEntry(*e) for e in entries])
         self._check(self.L.qk_set_schedule(self.h, arr, len(entries)))
 
+    def set_lsm_schedule(self, mode: int):
+        """LSM_AUTO, LSM_PER_DATE or LSM_COOPERATIVE for the backward pass."""
+        self._check(self.L.qk_set_lsm_schedule(self.h, mode))
+
     def lanes_per_thread(self, precision: int, scheme: int, n_lanes: int) -> int:
         return self.L.qk_lanes_per_thread(self.h, precision, scheme, n_lanes)
 

@@ -378,3 +378,20 @@ def test_american_other_bases_ikonen_toivanen(engine, basis, weighted):
                     kappa=5.0, theta=0.16, sigma=0.9, rho=0.1, price_scale=10.0)
     s = engine.price_american(c, qk.PUT, 1.0, basis=basis, weighted=weighted)
     assert abs(s.price - 0.520030) <= 3 * s.se + max(0.02 * 0.520030, 0.005), (s.price, s.se)
+
+
+@pytest.mark.parametrize("precision", [qk.FP32, qk.FP64])
+@pytest.mark.parametrize("basis,weighted,n_paths", [(qk.LAGUERRE_P03, False, 100_003), (qk.LAGUERRE_P04, True, 30_000),
+                                                    (qk.LAGUERRE_P02, False, 700)])
+def test_lsm_schedules_are_bit_identical(engine, precision, basis, weighted, n_paths):
+    """One cooperative launch for all dates and two launches per date: the same
+    partition, arithmetic and reduction order, so the same bits."""
+    c = base_contract(qk.BATES, qk.QE, precision, n_paths=n_paths, n_steps=40, price_scale=100.0)
+    out = {}
+    for mode in (qk._lib.LSM_PER_DATE, qk._lib.LSM_COOPERATIVE):
+        engine.set_lsm_schedule(mode)
+        out[mode] = engine.price_american(c, qk.PUT, 1.1, want_coeffs=True, basis=basis, weighted=weighted)
+    engine.set_lsm_schedule(qk._lib.LSM_AUTO)
+    (a, ca), (b, cb) = out[qk._lib.LSM_PER_DATE], out[qk._lib.LSM_COOPERATIVE]
+    assert (a.price, a.se, a.itm, a.mean, a.m2) == (b.price, b.se, b.itm, b.mean, b.m2)
+    assert np.array_equal(ca, cb)
