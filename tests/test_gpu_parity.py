@@ -395,3 +395,19 @@ def test_lsm_schedules_are_bit_identical(engine, precision, basis, weighted, n_p
     (a, ca), (b, cb) = out[qk._lib.LSM_PER_DATE], out[qk._lib.LSM_COOPERATIVE]
     assert (a.price, a.se, a.itm, a.mean, a.m2) == (b.price, b.se, b.itm, b.mean, b.m2)
     assert np.array_equal(ca, cb)
+
+
+def test_gpu_normal_moments(engine):
+    """GaussianRNGenTest.hpp:200-208 on the device stream: with constant variance
+    (kappa = theta = sigma = 0) each Euler log-increment is an exact affine image of
+    Z_S = rho Z1 + sqrt(1 - rho^2) Z2, so the normals are recovered from the simulated
+    paths: |mean| < 0.005, |var - 1| < 0.01 over 2M draws, no correlation between steps."""
+    v0, r, T, N = 0.04, 0.03, 1.0, 20
+    c = qe.contract(model=qk.HESTON, scheme=qk.EULER, precision=qk.FP64, n_steps=N, n_paths=100_000, seed=5,
+                    S0=1.0, v0=v0, r=r, q=0.0, T=T, kappa=0.0, theta=0.0, sigma=0.0, rho=-0.6)
+    S = engine.simulate_paths(c)
+    dt = T / N
+    logS = np.log(np.vstack([np.ones((1, c.n_paths)), S]))
+    Z = (np.diff(logS, axis=0) - (r - 0.5 * v0) * dt) / math.sqrt(v0 * dt)
+    assert abs(Z.mean()) < 0.005 and abs(Z.var() - 1.0) < 0.01, (Z.mean(), Z.var())
+    assert abs(np.corrcoef(Z[:-1].ravel(), Z[1:].ravel())[0, 1]) < 0.005
