@@ -317,7 +317,7 @@ def main():
     line = {
         "metric": METRIC, "value": v32, "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt32 * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "Heston Andersen-QE (psi_c=1.5, martingale correction), 4,000,000 paths x "
                                "365 steps, 3 payoffs from one path set: Asian call K=100, up-and-out "
                                "call K=100 B=130 daily, floating-strike lookback put",
@@ -329,7 +329,7 @@ def main():
                      "valu_instructions_per_path_step": _profile_value(
                          VALU_FILE, "valu_instructions_per_path_step_fp32", float),
                      "flops_per_path_step": FLOPS_PER_PS, "transcendentals_per_path_step": TRANS_PER_PS},
-        "fp64": {"value": v64, "ms_per_step": dt64 * 1e3 / s64, "kernel_ms": kavg64, "steps": s64,
+        "fp64": {"value": v64, "dtype": "f64", "ms_per_step": dt64 * 1e3 / s64, "kernel_ms": kavg64, "steps": s64,
                  "achieved_tflops": FLOPS_PER_PS * ps / (kavg64 * 1e-3) / 1e12,
                  "peak": PEAK_FP64_TFLOPS,
                  "valu_issue_util": _profile_value(VALU_FILE, "valu_issue_util_fp64", float)},
