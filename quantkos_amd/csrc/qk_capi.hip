@@ -299,10 +299,9 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     const qk_contract &c0 = lanes[0];
     const bool pathdep = std::any_of(pays, pays + n_pay, [](const qk_payoff &p) { return path_dependent(p.type); });
     const int L = qk_lanes_per_thread(ctx, c0.precision, c0.scheme, n_lanes);
-    const int n_padded = (n_lanes + L - 1) / L * L;
-    std::vector<LaneC<Real>> hl(n_padded);
+    std::vector<LaneC<Real>> hl(n_lanes);
     std::vector<double> hdisc(n_lanes), hscale(n_pay);
-    for (int i = 0; i < n_padded; ++i) hl[i] = make_lane<Real>(lanes[std::min(i, n_lanes - 1)]);
+    for (int i = 0; i < n_lanes; ++i) hl[i] = make_lane<Real>(lanes[i]);
     for (int i = 0; i < n_lanes; ++i) hdisc[i] = std::exp(-lanes[i].r * lanes[i].T);
     std::vector<PayD<Real>> hp(n_pay);
     for (int j = 0; j < n_pay; ++j) {
@@ -346,7 +345,22 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     a.pay_out = pay_out;
 
     QK_HIP(hipEventRecord(ctx->ev[0], s));
-    QK_HIP(launch_paths<Real>(a, c0.model, c0.scheme, pathdep, false, L, s));
+    // Whole groups of L lanes per thread; the lanes left over go to a launch
+    // with L/2 (then 1) lanes per thread instead of being padded.
+    for (int done = 0, Lk = L; done < n_lanes;) {
+        while (Lk > 1 && n_lanes - done < Lk) Lk >>= 1;
+        const int cnt = (n_lanes - done) / Lk * Lk;
+        PathArgs<Real> ak = a;
+        const int64_t off = (int64_t)done * n_pay * n_tiles;
+        ak.lanes = a.lanes + done;
+        ak.n_lanes = cnt;
+        ak.part_sum = a.part_sum + off;
+        ak.part_m2 = a.part_m2 + off;
+        ak.part_itm = a.part_itm + off;
+        ak.pay_out = done == 0 ? a.pay_out : nullptr;
+        QK_HIP(launch_paths<Real>(ak, c0.model, c0.scheme, pathdep, false, Lk, s));
+        done += cnt;
+    }
     QK_HIP(hipEventRecord(ctx->ev[1], s));
     QK_HIP(launch_merge(a.part_sum, a.part_m2, a.part_itm, n_tiles, c0.n_paths, n_lanes, n_pay,
                         (const double *)ctx->disc.p, (const double *)ctx->scale.p, ctx->mscratch.p,
