@@ -1,0 +1,99 @@
+"""The FP64 device polynomials (qk_device.hpp, Math<double>) against the
+functions they approximate, at 50 digits.
+
+The coefficients are read from the header as written, so a mistyped digit, a
+dropped term or a reversed order fails here.  The bounds are the near-minimax
+fit error plus the rounding of the coefficients to double:
+  log:  z P(z) vs 2 atanh(s)/s - 2 on z in [0, (3 - 2 sqrt 2)^2]    < 2e-17 absolute
+  sin:  S(w)   vs sin(pi/2 sqrt w)/sqrt w on w in [0, 1/4]          < 2.5e-16 absolute
+  cos:  C(w)   vs cos(pi/2 sqrt w)                                  < 6e-17 absolute
+  exp:  E(r)   vs e^r on r in [-0.35, 0.35]                         < 1e-17 relative
+and the split of ln 2 into LN2_HI (32 significant bits) + LN2_LO.
+"""
+import os
+import re
+
+import pytest
+
+mp = pytest.importorskip("mpmath")
+
+HEADER = os.path.join(os.path.dirname(__file__), "..", "quantkos_amd", "csrc", "qk_device.hpp")
+FIT = os.path.join(os.path.dirname(__file__), "..", "quantkos_amd", "csrc", "fit_fp64_poly.py")
+
+
+def header_array(name):
+    text = open(HEADER).read()
+    m = re.search(r"\b%s\[(\d+)\]\s*=\s*\{([^}]*)\}" % name, text)
+    assert m, name
+    vals = [float.fromhex(t.strip()) for t in m.group(2).split(",") if t.strip()]
+    assert len(vals) == int(m.group(1))
+    return vals
+
+
+def header_scalar(name):
+    m = re.search(r"\b%s\s*=\s*(-?0x[0-9a-fA-Fp.+-]+);" % name, open(HEADER).read())
+    assert m, name
+    return float.fromhex(m.group(1))
+
+
+def poly(c, x):
+    p = mp.mpf(c[-1])
+    for ci in reversed(c[:-1]):
+        p = p * x + mp.mpf(ci)
+    return p
+
+
+def grid(hi, n=1500):
+    return [hi * mp.mpf(i) / n for i in range(n + 1)]
+
+
+def test_log_polynomial():
+    mp.mp.dps = 50
+    c = header_array("LOG_P")
+    zmax = (3 - 2 * mp.sqrt(2)) ** 2
+    worst = 0
+    for z in grid(zmax)[1:]:
+        s = mp.sqrt(z)
+        exact = 2 * mp.atanh(s) / s - 2
+        worst = max(worst, abs(z * poly(c, z) - exact))
+    assert worst < 2e-17, worst
+
+
+def test_sin_cos_polynomials():
+    mp.mp.dps = 50
+    cs, cc = header_array("SIN_S"), header_array("COS_C")
+    ws, wc = 0, 0
+    for w in grid(mp.mpf(1) / 4)[1:]:
+        r = mp.sqrt(w)
+        ws = max(ws, abs(poly(cs, w) - mp.sin(mp.pi / 2 * r) / r))
+        wc = max(wc, abs(poly(cc, w) - mp.cos(mp.pi / 2 * r)))
+    assert ws < 2.5e-16, ws
+    assert wc < 6e-17, wc
+
+
+def test_exp_polynomial():
+    mp.mp.dps = 50
+    c = header_array("EXP_P")
+    worst = 0
+    for i in range(-1500, 1501):
+        r = mp.mpf("0.35") * i / 1500
+        worst = max(worst, abs(poly(c, r) / mp.exp(r) - 1))
+    assert worst < 1e-17, worst
+
+
+def test_ln2_split():
+    mp.mp.dps = 50
+    hi, lo = header_scalar("LN2_HI"), header_scalar("LN2_LO")
+    assert abs(mp.mpf(hi) + mp.mpf(lo) - mp.log(2)) < mp.mpf(2) ** -85
+    # k * LN2_HI exact for |k| <= 1100: the significand of LN2_HI fits in 32 bits
+    m, e = mp.frexp(mp.mpf(hi))
+    assert (m * 2 ** 32) == int(m * 2 ** 32)
+
+
+def test_fit_script_reproduces_header():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("fit_fp64_poly", FIT)
+    fit = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(fit)
+    for name in ("LOG_P", "SIN_S", "COS_C", "EXP_P"):
+        assert fit.coefficients(name) == header_array(name), name
