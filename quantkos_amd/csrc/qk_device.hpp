@@ -132,17 +132,20 @@ template <> struct Math<double> {
         r = __builtin_fma(-n, LN2_LO, r);
         return __builtin_amdgcn_ldexp(horner(EXP_P, r), (int)n);
     }
-    // Natural log of a positive finite x, by the reduction of Sun's fdlibm
-    // e_log.c: x = 2^k m, m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact),
+    // Natural log of a positive normal finite x, by the reduction of Sun's
+    // fdlibm e_log.c: x = 2^k m, m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact),
     // s = f/(2+f), log(1+f) = 2 atanh(s) = f - (h - s (h + z P(z))), h = f^2/2,
-    // z = s^2.  s is f times v_rcp_f64 (relative error ~2^-23) refined by two
-    // residual steps; the result is k LN2_HI + (f - (h - (s (h + R) + k LN2_LO))).
+    // z = s^2.  k and m come from the high word: subtracting the high word of
+    // sqrt(1/2) puts k in its exponent field, and removing k from the exponent
+    // of x leaves m.  s is f times v_rcp_f64 (relative error ~2^-23) refined by
+    // two residual steps; the result is k LN2_HI + (f - (h - (s (h + R) + k LN2_LO))).
     __device__ static __forceinline__ double log(double x) {
-        double m = __builtin_amdgcn_frexp_mant(x);              // [1/2, 1)
-        int k = __builtin_amdgcn_frexp_exp(x);
-        const bool lo = m < 0x1.6a09e667f3bcdp-1;
-        m = lo ? m + m : m;
-        k = lo ? k - 1 : k;
+        const uint64_t ix = __builtin_bit_cast(uint64_t, x);
+        const uint32_t hx = (uint32_t)(ix >> 32);
+        const int32_t tmp = (int32_t)(hx - 0x3fe6a09eu);
+        const int k = tmp >> 20;
+        const uint32_t hm = hx - ((uint32_t)tmp & 0xfff00000u);
+        const double m = __builtin_bit_cast(double, ((uint64_t)hm << 32) | (uint32_t)ix);
         const double f = m - 1.0;
         const double d = 2.0 + f;
         const double r0 = __builtin_amdgcn_rcp(d);
@@ -197,9 +200,11 @@ template <> struct Math<double> {
         s = (q & 2) ? -a : a;
         c = ((q + 1) & 2) ? -b : b;
     }
-    // All 32 bits of a Philox word, centred: (w + 0.5) * 2^-32, exact in FP64.
+    // All 32 bits of a Philox word, centred: (w + 0.5) * 2^-32, exact in FP64
+    // (0.5 is an inline operand and 2^-32 a 32-bit literal; 2^-33 as the
+    // addend of an fma would take two moves per draw).
     __device__ static __forceinline__ double uniform(uint32_t w) {
-        return __builtin_fma((double)w, 0x1.0p-32, 0x1.0p-33);
+        return ((double)w + 0.5) * 0x1.0p-32;
     }
 };
 
