@@ -383,6 +383,33 @@ __device__ __forceinline__ Real payoff_eval(const PayD<Real> &p, Real S, double 
 }
 
 // ---------------------------------------------------------------------------
+// Raw buffer access.  A gfx9 buffer resource (base address, no stride,
+// num_records in bytes, dword 3 = 0x00020000) is uniform, so the address is
+// a 32-bit byte offset in one vector register plus the instruction's offset
+// field; a load at an offset >= num_records returns 0 and a store there is
+// dropped, so reads past the end of a range need no branch or clamp.
+// ---------------------------------------------------------------------------
+typedef uint32_t u32x2_t __attribute__((__vector_size__(8)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename Real> __device__ __forceinline__ Real buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, off, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
 // Wavefront (64-lane) reductions.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double x) {

@@ -351,9 +351,9 @@ __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpar
 // date t-1 in FP64 and stores the group's sums in part[0 .. STRIDE) (nothing
 // if part is null); (3) at date 0 applies the final one-step discount and
 // writes the per-tile moments of the cash flows.  cfp[i - p0] is the cash flow
-// of path i (device or LDS memory).  Holds a block barrier when t >= 1: every
+// of path i (device memory, or LDS when CF_LDS).  Holds a block barrier when t >= 1: every
 // thread of the block must call it for the same t.
-template <typename Real, int NB, bool W>
+template <typename Real, int NB, bool W, bool CF_LDS>
 __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, Real *cfp, int64_t n, int64_t p0,
                                                int64_t p1, int t, int N, int right, Real K, Real disc, Real itm_eps,
                                                const double C[NB], double *part, double *psum, double *pm2,
@@ -364,21 +364,57 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
     double g[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) g[k] = 0.0;
-    constexpr int U = 8;    // 256-path rows per pass; their loads are issued together
-    for (int64_t base = p0; base < p1; base += 256 * U) {
+    // 256-path rows per pass; a pass's loads are issued together, and the
+    // next pass's loads are issued before this pass is computed (they are
+    // other paths, so the cash-flow writes of this pass do not touch them).
+    // Paths are addressed by 32-bit offsets o = i - p0 through buffer
+    // resources over [p0, p1) of the date's row: rows past p1 read 0 without a
+    // branch (they are then masked out).  Cash flows in LDS (CF_LDS) are read
+    // with a bounds test instead.
+    constexpr int U = 8;
+    constexpr uint32_t SZ = sizeof(Real);
+    const uint32_t len = (uint32_t)(p1 - p0);
+    const __amdgpu_buffer_rsrc_t r_s0 = buf_rsrc(store + (int64_t)t * n + p0, len * SZ);
+    const __amdgpu_buffer_rsrc_t r_s1 = buf_rsrc(store + (int64_t)(t >= 1 ? t - 1 : 0) * n + p0, len * SZ);
+    const __amdgpu_buffer_rsrc_t r_cf = buf_rsrc(cfp, CF_LDS ? 0u : len * SZ);
+    Real Sn[U], S1n[U], cfn[U];
+    auto fetch = [&](uint32_t r0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) Sn[u] = buf_load<Real>(r_s0, (r0 + (uint32_t)(u * 256 + lt)) * SZ);
+        if (t >= 1) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) S1n[u] = buf_load<Real>(r_s1, (r0 + (uint32_t)(u * 256 + lt)) * SZ);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) S1n[u] = Real(0);
+        }
+        if (t < N - 1) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t o = r0 + (uint32_t)(u * 256 + lt);
+                if constexpr (CF_LDS) cfn[u] = o < len ? cfp[o] : Real(0);
+                else cfn[u] = buf_load<Real>(r_cf, o * SZ);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) cfn[u] = Real(0);
+        }
+    };
+    if (len > 0) fetch(0);
+    for (uint32_t r0 = 0; r0 < len; r0 += 256u * U) {
         Real Sv[U], S1v[U], cfv[U];
         bool act[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t i = base + u * 256 + lt;
-            act[u] = i < p1;
-            Sv[u] = act[u] ? store[(int64_t)t * n + i] : Real(0);
-            S1v[u] = (act[u] && t >= 1) ? store[(int64_t)(t - 1) * n + i] : Real(0);
-            cfv[u] = (act[u] && t < N - 1) ? cfp[i - p0] : Real(0);
+            Sv[u] = Sn[u];
+            S1v[u] = S1n[u];
+            cfv[u] = cfn[u];
+            act[u] = r0 + (uint32_t)(u * 256 + lt) < len;
         }
+        if (r0 + 256u * U < len) fetch(r0 + 256u * U);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t i = base + u * 256 + lt;
+            const uint32_t o = r0 + (uint32_t)(u * 256 + lt);
             Real c = Real(0);
             if (act[u]) {
                 const Real Sx = Sv[u];
@@ -417,7 +453,8 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
                 } else {
                     c = c * disc;   // LSMAmerican.hpp:393-405
                 }
-                cfp[i - p0] = c;
+                if constexpr (CF_LDS) cfp[o] = c;
+                else buf_store(c, r_cf, o * SZ);
             }
             if (t == 0) {
                 const unsigned long long am = __ballot(act[u]);
@@ -430,7 +467,7 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
                     const double m2 = wave_sum(d * d);
                     const unsigned long long im = __ballot(act[u] && c > itm_eps);
                     if (lid == 0) {
-                        const int64_t tile = (base + u * 256) / TILE + wv;
+                        const int64_t tile = (p0 + (int64_t)r0 + u * 256) / TILE + wv;
                         psum[tile] = sx;
                         pm2[tile] = m2;
                         pitm[tile] = (uint32_t)__popcll(im);
@@ -510,7 +547,7 @@ __global__ __launch_bounds__(256) void k_lsm_step(PumpArgs<Real> a) {
     const int64_t p0 = (int64_t)b * q.chunk;
     const int64_t p1 = p0 + q.chunk < q.n_paths ? p0 + q.chunk : q.n_paths;
     double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * S::STRIDE + (int64_t)b * S::STRIDE : nullptr;
-    lsm_chunk_date<Real, NB, W>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
+    lsm_chunk_date<Real, NB, W, false>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
                                 part, q.psum, q.pm2, q.pitm, wpart);
 }
 
@@ -606,7 +643,7 @@ __global__ __launch_bounds__(256 * LSM_COOP_SUBS) void k_lsm_coop(LsmCoopArgs<Re
         for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? Cs[k] : 0.0;
         const int par = (t - 1) & 1;
         double *part = (t >= 1 && live) ? a.bpart + (int64_t)par * a.nb * S::STRIDE + (int64_t)vb * S::STRIDE : nullptr;
-        lsm_chunk_date<Real, NB, W>(a.store, cfp, a.n_paths, p0, p1, t, N, a.right, a.K, a.disc, a.itm_eps, C, part,
+        lsm_chunk_date<Real, NB, W, true>(a.store, cfp, a.n_paths, p0, p1, t, N, a.right, a.K, a.disc, a.itm_eps, C, part,
                                     a.psum, a.pm2, a.pitm, wpart[sub]);
         if (t >= 1) {
             ++k_bar;
