@@ -196,7 +196,8 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
         b += (size_t)c.n_paths * (size_t)c.n_steps * rs;   // path store [date][path]
         b += (size_t)c.n_paths * rs;                         // cash flows
         int64_t chunk = 0;
-        b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 + (size_t)2 * pump_blocks(c.n_paths, &chunk) * LSM_STRIDE * 8;
+        const int group = c.precision == QK_FP32 ? lsm_group<float>() : lsm_group<double>();
+        b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 + (size_t)2 * pump_blocks(c.n_paths, group, &chunk) * LSM_STRIDE * 8;
     }
     return b;
 }
@@ -560,7 +561,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     const int N = c.n_steps;
     const int64_t n_tiles = (n + TILE - 1) / TILE;
     int64_t chunk = 0;
-    const int nb = pump_blocks(n, &chunk);
+    const int nb = pump_blocks(n, lsm_group<Real>(), &chunk);
     if ((st = ctx->cf.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
     if ((st = ctx->coeffs.reserve(ctx, (size_t)N * n_basis * 8)) != QK_OK) return st;
     if ((st = ctx->bpart.reserve(ctx, (size_t)2 * nb * LSM_STRIDE * 8)) != QK_OK) return st;
@@ -600,15 +601,15 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     // (when the cash flows of a block's chunks fit its LDS and every block can
     // be resident).  Both give bit-identical results; the per-date launches
     // measure faster on MI355X, so they are the default.
-    const int coop_blocks = (nb + LSM_COOP_SUBS - 1) / LSM_COOP_SUBS;
-    const size_t coop_lds = (size_t)LSM_COOP_SUBS * chunk * sizeof(Real);
+    const int coop_blocks = nb;
+    const size_t coop_lds = (size_t)chunk * sizeof(Real);
     bool coop = false;
     if (ctx->lsm_mode == QK_LSM_COOPERATIVE) {
         int max_lds = 0, n_cu = 0, coop_ok = 0;
         (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
         (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
         (void)hipDeviceGetAttribute(&coop_ok, hipDeviceAttributeCooperativeLaunch, ctx->device);
-        coop = coop_ok && coop_lds + 8192 <= (size_t)max_lds &&
+        coop = coop_ok && coop_lds + 16384 <= (size_t)max_lds &&
                (int64_t)lsm_coop_occupancy<Real>(n_basis, weighted, coop_lds) * n_cu >= coop_blocks;
     }
     if (coop) {

@@ -305,26 +305,26 @@ __device__ void lsm_solve(const double *s, double C[NB]) {
 }
 
 // Sums the block partials [nb][STRIDE] of one date's regression in a fixed
-// order and solves the NB x NB normal equations in FP64 (256 threads; the
-// partials are read as one coalesced stream: STRIDE divides 256, so thread i
+// order and solves the NB x NB normal equations in FP64 (G threads; the
+// partials are read as one coalesced stream: STRIDE divides G, so thread i
 // always meets component i % STRIDE).  Every thread of the block must call it
 // (it holds block barriers); only `active` threads read, and thread 0 of an
 // active group leaves the coefficients in C.
-template <int NB>
+template <int NB, int G>
 __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpart, int nb, double *red, bool active,
                                                  double C[NB]) {
     using S = LsmSums<NB>;
     constexpr int NS = S::NS, ST = S::STRIDE;
-    const int lt = threadIdx.x & 255;
+    const int lt = threadIdx.x & (G - 1);
     constexpr int UR = 16;  // loads in flight per thread; partial sums combined in a fixed order
     double acc[UR];
 #pragma unroll
     for (int u = 0; u < UR; ++u) acc[u] = 0.0;
     const int64_t total = active ? (int64_t)nb * ST : 0;
-    for (int64_t e0 = lt; e0 < total; e0 += 256 * UR) {
+    for (int64_t e0 = lt; e0 < total; e0 += G * UR) {
 #pragma unroll
         for (int u = 0; u < UR; ++u) {
-            const int64_t e = e0 + (int64_t)u * 256;
+            const int64_t e = e0 + (int64_t)u * G;
             if (e < total) acc[u] += bpart[e];
         }
     }
@@ -333,7 +333,7 @@ __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpar
     for (int u = 0; u < UR; ++u) tot += acc[u];
     if (active) red[lt] = tot;
     __syncthreads();
-    for (int s = 128; s >= ST; s >>= 1) {
+    for (int s = G / 2; s >= ST; s >>= 1) {
         if (active && lt < s) red[lt] += red[lt + s];
         __syncthreads();
     }
@@ -344,8 +344,8 @@ __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpar
     }
 }
 
-// One Longstaff-Schwartz date t over the paths [p0, p1) by a group of 256
-// threads (threadIdx.x & 255 is the thread's place in the group): (1) updates
+// One Longstaff-Schwartz date t over the paths [p0, p1) by a group of G
+// threads (threadIdx.x & (G - 1) is the thread's place in the group): (1) updates
 // the cash flows at date t with the coefficients C of date t (initialises them
 // at the last date); (2) for t >= 1 accumulates the in-the-money Gram sums of
 // date t-1 in FP64 and stores the group's sums in part[0 .. STRIDE) (nothing
@@ -353,18 +353,18 @@ __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpar
 // writes the per-tile moments of the cash flows.  cfp[i - p0] is the cash flow
 // of path i (device memory, or LDS when CF_LDS).  Holds a block barrier when t >= 1: every
 // thread of the block must call it for the same t.
-template <typename Real, int NB, bool W, bool CF_LDS>
+template <typename Real, int NB, bool W, bool CF_LDS, int G>
 __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, Real *cfp, int64_t n, int64_t p0,
                                                int64_t p1, int t, int N, int right, Real K, Real disc, Real itm_eps,
                                                const double C[NB], double *part, double *psum, double *pm2,
                                                uint32_t *pitm, double (*wpart)[LsmSums<NB>::NS]) {
     using S = LsmSums<NB>;
     constexpr int NS = S::NS;
-    const int lt = threadIdx.x & 255, lid = lt & 63, wv = lt >> 6;
+    const int lt = threadIdx.x & (G - 1), lid = lt & 63, wv = lt >> 6;
     double g[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) g[k] = 0.0;
-    // 256-path rows per pass; a pass's loads are issued together, and the
+    // G-path rows per pass; a pass's loads are issued together, and the
     // next pass's loads are issued before this pass is computed (they are
     // other paths, so the cash-flow writes of this pass do not touch them).
     // Paths are addressed by 32-bit offsets o = i - p0 through buffer
@@ -380,10 +380,10 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
     Real Sn[U], S1n[U], cfn[U];
     auto fetch = [&](uint32_t r0) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) Sn[u] = buf_load<Real>(r_s0, (r0 + (uint32_t)(u * 256 + lt)) * SZ);
+        for (int u = 0; u < U; ++u) Sn[u] = buf_load<Real>(r_s0, (r0 + (uint32_t)(u * G + lt)) * SZ);
         if (t >= 1) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) S1n[u] = buf_load<Real>(r_s1, (r0 + (uint32_t)(u * 256 + lt)) * SZ);
+            for (int u = 0; u < U; ++u) S1n[u] = buf_load<Real>(r_s1, (r0 + (uint32_t)(u * G + lt)) * SZ);
         } else {
 #pragma unroll
             for (int u = 0; u < U; ++u) S1n[u] = Real(0);
@@ -391,7 +391,7 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
         if (t < N - 1) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t o = r0 + (uint32_t)(u * 256 + lt);
+                const uint32_t o = r0 + (uint32_t)(u * G + lt);
                 if constexpr (CF_LDS) cfn[u] = o < len ? cfp[o] : Real(0);
                 else cfn[u] = buf_load<Real>(r_cf, o * SZ);
             }
@@ -401,7 +401,7 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
         }
     };
     if (len > 0) fetch(0);
-    for (uint32_t r0 = 0; r0 < len; r0 += 256u * U) {
+    for (uint32_t r0 = 0; r0 < len; r0 += (uint32_t)G * U) {
         Real Sv[U], S1v[U], cfv[U];
         bool act[U];
 #pragma unroll
@@ -409,12 +409,12 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
             Sv[u] = Sn[u];
             S1v[u] = S1n[u];
             cfv[u] = cfn[u];
-            act[u] = r0 + (uint32_t)(u * 256 + lt) < len;
+            act[u] = r0 + (uint32_t)(u * G + lt) < len;
         }
-        if (r0 + 256u * U < len) fetch(r0 + 256u * U);
+        if (r0 + (uint32_t)G * U < len) fetch(r0 + (uint32_t)G * U);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t o = r0 + (uint32_t)(u * 256 + lt);
+            const uint32_t o = r0 + (uint32_t)(u * G + lt);
             Real c = Real(0);
             if (act[u]) {
                 const Real Sx = Sv[u];
@@ -467,7 +467,7 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
                     const double m2 = wave_sum(d * d);
                     const unsigned long long im = __ballot(act[u] && c > itm_eps);
                     if (lid == 0) {
-                        const int64_t tile = (p0 + (int64_t)r0 + u * 256) / TILE + wv;
+                        const int64_t tile = (p0 + (int64_t)r0 + u * G) / TILE + wv;
                         psum[tile] = sx;
                         pm2[tile] = m2;
                         pitm[tile] = (uint32_t)__popcll(im);
@@ -507,7 +507,7 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
         if (part != nullptr && lt < S::STRIDE) {          // the padding components are written as zero
             double sx = 0.0;
             if (lt < NS)
-                for (int w = 0; w < WAVES_PER_BLOCK; ++w) sx += wpart[w][lt];
+                for (int w = 0; w < G / 64; ++w) sx += wpart[w][lt];
             part[lt] = sx;
         }
     }
@@ -521,19 +521,20 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
 // t-1 into buffer (t-1) & 1.  The redundant solves read nb partials per block;
 // nb is kept at most PUMP_MAX_BLOCKS for that reason.
 template <typename Real, int NB, bool W>
-__global__ __launch_bounds__(256) void k_lsm_step(PumpArgs<Real> a) {
+__global__ __launch_bounds__(lsm_group<Real>()) void k_lsm_step(PumpArgs<Real> a) {
     using S = LsmSums<NB>;
+    constexpr int G = lsm_group<Real>();
     int r = 0;
     while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
     const PumpReq<Real> &q = a.r[r];
     const int b = (int)blockIdx.x - a.block_start[r];
     const int t = q.t, N = q.n_steps;
-    __shared__ double red[256];
+    __shared__ double red[G];
     __shared__ double Cs[NB];
-    __shared__ double wpart[WAVES_PER_BLOCK][S::NS];
+    __shared__ double wpart[G / 64][S::NS];
     if (t < N - 1) {
         double Cn[NB];
-        lsm_reduce_solve<NB>(q.bpart + (int64_t)(t & 1) * q.nb * S::STRIDE, q.nb, red, true, Cn);
+        lsm_reduce_solve<NB, G>(q.bpart + (int64_t)(t & 1) * q.nb * S::STRIDE, q.nb, red, true, Cn);
         if (threadIdx.x == 0)
             for (int k = 0; k < NB; ++k) {
                 Cs[k] = Cn[k];
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(256) void k_lsm_step(PumpArgs<Real> a) {
     const int64_t p0 = (int64_t)b * q.chunk;
     const int64_t p1 = p0 + q.chunk < q.n_paths ? p0 + q.chunk : q.n_paths;
     double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * S::STRIDE + (int64_t)b * S::STRIDE : nullptr;
-    lsm_chunk_date<Real, NB, W, false>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
+    lsm_chunk_date<Real, NB, W, false, G>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
                                 part, q.psum, q.pm2, q.pitm, wpart);
 }
 
@@ -612,29 +613,28 @@ __device__ __forceinline__ bool coop_wait(unsigned int *sync, unsigned int k) {
     return ok != 0;
 }
 
-// The whole backward pass of one request in one cooperative launch: block g
-// holds the chunks of virtual blocks LSM_COOP_SUBS*g + s (s = its 256-thread
-// groups), the same partition, arithmetic and reduction order as the
-// launch-per-date k_lsm_step (so the results are bit-identical), with the
-// cash flows in LDS.  Per date: partials, arrival, and either the solve (the
-// block that completes the count) or the wait for its published
-// coefficients.  Partials and published coefficients alternate between two
-// buffers by date parity.
+// The whole backward pass of one request in one cooperative launch: block b
+// holds the chunk of pump block b, the same partition, arithmetic and
+// reduction order as the launch-per-date k_lsm_step (so the results are
+// bit-identical), with the cash flows in LDS.  Per date: partials, arrival,
+// and either the solve (the block that completes the count) or the wait for
+// its published coefficients.  Partials and published coefficients alternate
+// between two buffers by date parity.
 template <typename Real, int NB, bool W>
-__global__ __launch_bounds__(256 * LSM_COOP_SUBS) void k_lsm_coop(LsmCoopArgs<Real> a) {
+__global__ __launch_bounds__(lsm_group<Real>()) void k_lsm_coop(LsmCoopArgs<Real> a) {
     using S = LsmSums<NB>;
+    constexpr int G = lsm_group<Real>();
     extern __shared__ unsigned char lds_raw[];
     Real *cf_lds = reinterpret_cast<Real *>(lds_raw);
-    __shared__ double wpart[LSM_COOP_SUBS][WAVES_PER_BLOCK][S::NS];
-    __shared__ double red[256];
+    __shared__ double wpart[G / 64][S::NS];
+    __shared__ double red[G];
     __shared__ double Cs[NB];
     double *cpub = reinterpret_cast<double *>(reinterpret_cast<unsigned char *>(a.sync) + SYNC_COEF_BYTES);
-    const int sub = threadIdx.x >> 8;
-    const int vb = (int)blockIdx.x * LSM_COOP_SUBS + sub;
+    const int vb = (int)blockIdx.x;
     const bool live = vb < a.nb;
     const int64_t p0 = live ? (int64_t)vb * a.chunk : 0;
     const int64_t p1 = live ? (p0 + a.chunk < a.n_paths ? p0 + a.chunk : a.n_paths) : 0;
-    Real *cfp = cf_lds + (int64_t)sub * a.chunk;
+    Real *cfp = cf_lds;
     const int N = a.n_steps;
     unsigned int k_bar = 0;
     for (int t = N - 1; t >= 0; --t) {
@@ -643,13 +643,13 @@ __global__ __launch_bounds__(256 * LSM_COOP_SUBS) void k_lsm_coop(LsmCoopArgs<Re
         for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? Cs[k] : 0.0;
         const int par = (t - 1) & 1;
         double *part = (t >= 1 && live) ? a.bpart + (int64_t)par * a.nb * S::STRIDE + (int64_t)vb * S::STRIDE : nullptr;
-        lsm_chunk_date<Real, NB, W, true>(a.store, cfp, a.n_paths, p0, p1, t, N, a.right, a.K, a.disc, a.itm_eps, C, part,
-                                    a.psum, a.pm2, a.pitm, wpart[sub]);
+        lsm_chunk_date<Real, NB, W, true, G>(a.store, cfp, a.n_paths, p0, p1, t, N, a.right, a.K, a.disc, a.itm_eps, C,
+                                             part, a.psum, a.pm2, a.pitm, wpart);
         if (t >= 1) {
             ++k_bar;
             if (coop_arrive(a.sync, k_bar)) {
                 double Cn[NB];
-                lsm_reduce_solve<NB>(a.bpart + (int64_t)par * a.nb * S::STRIDE, a.nb, red, sub == 0, Cn);
+                lsm_reduce_solve<NB, G>(a.bpart + (int64_t)par * a.nb * S::STRIDE, a.nb, red, true, Cn);
                 if (threadIdx.x == 0) {
                     for (int k = 0; k < NB; ++k) {
                         Cs[k] = Cn[k];
@@ -666,7 +666,7 @@ __global__ __launch_bounds__(256 * LSM_COOP_SUBS) void k_lsm_coop(LsmCoopArgs<Re
         }
     }
     // Final cash flows to device memory (payoff distribution, path-set users).
-    for (int64_t i = p0 + (threadIdx.x & 255); i < p1; i += 256) a.cf[i] = cfp[i - p0];
+    for (int64_t i = p0 + threadIdx.x; i < p1; i += G) a.cf[i] = cfp[i - p0];
 }
 
 
@@ -872,8 +872,8 @@ template hipError_t launch_select_hist<float>(const float *, int64_t, const Sele
 template hipError_t launch_select_hist<double>(const double *, int64_t, const SelectArgs &, uint32_t *, int,
                                                hipStream_t);
 
-int pump_blocks(int64_t n_paths, int64_t *chunk) {
-    const int64_t nb = std::min<int64_t>((n_paths + 255) / 256, PUMP_MAX_BLOCKS);
+int pump_blocks(int64_t n_paths, int group, int64_t *chunk) {
+    const int64_t nb = std::min<int64_t>((n_paths + group - 1) / group, PUMP_MAX_BLOCKS);
     const int64_t c = ((n_paths + nb - 1) / nb + 255) / 256 * 256;
     *chunk = c;
     return (int)((n_paths + c - 1) / c);
@@ -881,21 +881,21 @@ int pump_blocks(int64_t n_paths, int64_t *chunk) {
 
 template <typename Real, int NB, bool W>
 static void launch_pump_nb(const PumpArgs<Real> &a, hipStream_t s) {
-    hipLaunchKernelGGL((k_lsm_step<Real, NB, W>), dim3(a.block_start[a.n_req]), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_lsm_step<Real, NB, W>), dim3(a.block_start[a.n_req]), dim3(lsm_group<Real>()), 0, s, a);
 }
 
 template <typename Real, int NB, bool W>
 static hipError_t launch_coop_nb(const LsmCoopArgs<Real> &a, int blocks, size_t lds, hipStream_t s) {
     void *args[] = {const_cast<LsmCoopArgs<Real> *>(&a)};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_lsm_coop<Real, NB, W>), dim3(blocks),
-                                      dim3(256 * LSM_COOP_SUBS), args, (unsigned int)lds, s);
+                                      dim3(lsm_group<Real>()), args, (unsigned int)lds, s);
 }
 
 template <typename Real, int NB, bool W>
 static int coop_occupancy(size_t lds) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_lsm_coop<Real, NB, W>),
-                                                     256 * LSM_COOP_SUBS, lds) != hipSuccess)
+                                                     lsm_group<Real>(), lds) != hipSuccess)
         return 0;
     return per_cu;
 }

@@ -59,9 +59,14 @@ template <typename Real>
 hipError_t launch_select_hist(const Real *vals, int64_t n, const SelectArgs &a, uint32_t *hist, int blocks,
                               hipStream_t s);
 
+// Threads per Longstaff-Schwartz block (k_lsm_step, k_lsm_coop): one block
+// per chunk of paths and one Gram partial per block.  16 wavefronts (FP32) or
+// 8 (FP64, whose date kernel needs up to 256 registers) per block keep 4 or 2
+// wavefronts on every SIMD when the 256 partials' blocks cover the GPU.
+template <typename Real>
+constexpr int lsm_group() { return sizeof(Real) == 4 ? 1024 : 512; }
 // One request's whole backward pass in one cooperative launch (k_lsm_coop):
-// LSM_COOP_SUBS groups of 256 threads per block, one pump "virtual block" each.
-constexpr int LSM_COOP_SUBS = 2;
+// one block of lsm_group<Real>() threads per chunk.
 constexpr unsigned int LSM_COOP_SPIN_LIMIT = 1u << 22;   // polls per wait before giving up
 constexpr int LSM_COOP_SYNC_ERR_WORD = 16 * 8 + 32;       // index of the error word in LsmCoopArgs::sync
 template <typename Real>
@@ -83,8 +88,9 @@ template <typename Real>
 hipError_t launch_lsm_coop(const LsmCoopArgs<Real> &a, int n_basis, bool weighted, int blocks, size_t lds,
                            hipStream_t s);
 
-// Blocks and paths per block of one request's pump work.
-int pump_blocks(int64_t n_paths, int64_t *chunk);
+// Blocks and paths per block of one request's pump work (group = threads per
+// block, lsm_group<Real>()).
+int pump_blocks(int64_t n_paths, int group, int64_t *chunk);
 
 // A request of a serving batch (one payoff, or a path store for LSM).
 constexpr int32_t REQ_PATHDEP = 1, REQ_STORE = 2;
