@@ -196,8 +196,9 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
         b += (size_t)c.n_paths * (size_t)c.n_steps * rs;   // path store [date][path]
         b += (size_t)c.n_paths * rs;                         // cash flows
         int64_t chunk = 0;
-        const int group = c.precision == QK_FP32 ? lsm_group<float>() : lsm_group<double>();
-        b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 + (size_t)2 * pump_blocks(c.n_paths, group, &chunk) * LSM_STRIDE * 8;
+        // Gram partials: the smaller block size gives the larger block count.
+        b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 +
+             (size_t)2 * pump_blocks(c.n_paths, LSM_SHARED_GROUP, &chunk) * LSM_STRIDE * 8;
     }
     return b;
 }
@@ -635,7 +636,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     } else {
         for (int t = N - 1; t >= 0; --t) {               // ForwardBackwardMC.hpp:219-242
             q.t = t;
-            QK_HIP(launch_pump<Real>(a, n_basis, weighted, s));
+            QK_HIP(launch_pump<Real>(a, n_basis, weighted, lsm_group<Real>(), s));
         }
     }
     QK_HIP(hipEventRecord(ctx->ev[3], s));

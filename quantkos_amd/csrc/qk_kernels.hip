@@ -520,10 +520,9 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
 // then runs the date over its `chunk` of paths and writes its partial of date
 // t-1 into buffer (t-1) & 1.  The redundant solves read nb partials per block;
 // nb is kept at most PUMP_MAX_BLOCKS for that reason.
-template <typename Real, int NB, bool W>
-__global__ __launch_bounds__(lsm_group<Real>()) void k_lsm_step(PumpArgs<Real> a) {
+template <typename Real, int NB, bool W, int G>
+__global__ __launch_bounds__(G) void k_lsm_step(PumpArgs<Real> a) {
     using S = LsmSums<NB>;
-    constexpr int G = lsm_group<Real>();
     int r = 0;
     while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
     const PumpReq<Real> &q = a.r[r];
@@ -880,8 +879,11 @@ int pump_blocks(int64_t n_paths, int group, int64_t *chunk) {
 }
 
 template <typename Real, int NB, bool W>
-static void launch_pump_nb(const PumpArgs<Real> &a, hipStream_t s) {
-    hipLaunchKernelGGL((k_lsm_step<Real, NB, W>), dim3(a.block_start[a.n_req]), dim3(lsm_group<Real>()), 0, s, a);
+static void launch_pump_nb(const PumpArgs<Real> &a, int group, hipStream_t s) {
+    constexpr int GB = lsm_group<Real>();
+    if (group == GB) hipLaunchKernelGGL((k_lsm_step<Real, NB, W, GB>), dim3(a.block_start[a.n_req]), dim3(GB), 0, s, a);
+    else hipLaunchKernelGGL((k_lsm_step<Real, NB, W, LSM_SHARED_GROUP>), dim3(a.block_start[a.n_req]),
+                            dim3(LSM_SHARED_GROUP), 0, s, a);
 }
 
 template <typename Real, int NB, bool W>
@@ -932,15 +934,16 @@ template hipError_t launch_lsm_coop<float>(const LsmCoopArgs<float> &, int, bool
 template hipError_t launch_lsm_coop<double>(const LsmCoopArgs<double> &, int, bool, int, size_t, hipStream_t);
 
 template <typename Real>
-hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s) {
+hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, int group, hipStream_t s) {
     if (a.n_req <= 0) return hipSuccess;
+    if (group != lsm_group<Real>() && group != LSM_SHARED_GROUP) return hipErrorInvalidValue;
     switch (n_basis * 2 + (weighted ? 1 : 0)) {
-    case 6: launch_pump_nb<Real, 3, false>(a, s); break;
-    case 7: launch_pump_nb<Real, 3, true>(a, s); break;
-    case 8: launch_pump_nb<Real, 4, false>(a, s); break;
-    case 9: launch_pump_nb<Real, 4, true>(a, s); break;
-    case 10: launch_pump_nb<Real, 5, false>(a, s); break;
-    case 11: launch_pump_nb<Real, 5, true>(a, s); break;
+    case 6: launch_pump_nb<Real, 3, false>(a, group, s); break;
+    case 7: launch_pump_nb<Real, 3, true>(a, group, s); break;
+    case 8: launch_pump_nb<Real, 4, false>(a, group, s); break;
+    case 9: launch_pump_nb<Real, 4, true>(a, group, s); break;
+    case 10: launch_pump_nb<Real, 5, false>(a, group, s); break;
+    case 11: launch_pump_nb<Real, 5, true>(a, group, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -967,7 +970,7 @@ template hipError_t launch_batch<double>(const ReqD<double> *, const ItemD *, in
                                          hipStream_t);
 template hipError_t launch_paths<float>(const PathArgs<float> &, int, int, bool, bool, int, hipStream_t);
 template hipError_t launch_paths<double>(const PathArgs<double> &, int, int, bool, bool, int, hipStream_t);
-template hipError_t launch_pump<float>(const PumpArgs<float> &, int, bool, hipStream_t);
-template hipError_t launch_pump<double>(const PumpArgs<double> &, int, bool, hipStream_t);
+template hipError_t launch_pump<float>(const PumpArgs<float> &, int, bool, int, hipStream_t);
+template hipError_t launch_pump<double>(const PumpArgs<double> &, int, bool, int, hipStream_t);
 
 }  // namespace qk

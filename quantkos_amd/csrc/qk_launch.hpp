@@ -65,6 +65,11 @@ hipError_t launch_select_hist(const Real *vals, int64_t n, const SelectArgs &a, 
 // wavefronts on every SIMD when the 256 partials' blocks cover the GPU.
 template <typename Real>
 constexpr int lsm_group() { return sizeof(Real) == 4 ? 1024 : 512; }
+// Threads per block of the request server's pump, which shares the GPU with
+// batch launches on other streams: smaller blocks find room on partly
+// occupied CUs (measured: 1,024-thread pump blocks cut the config-4 burst rate
+// by 13 %).
+constexpr int LSM_SHARED_GROUP = 256;
 // One request's whole backward pass in one cooperative launch (k_lsm_coop):
 // one block of lsm_group<Real>() threads per chunk.
 constexpr unsigned int LSM_COOP_SPIN_LIMIT = 1u << 22;   // polls per wait before giving up
@@ -134,6 +139,9 @@ hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pit
                         int n_lanes, int n_pay, const double *disc, const double *scale, void *scratch, qk_stats *out,
                         hipStream_t s);
 hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s);
+// One k_lsm_step launch; group = threads per block, lsm_group<Real>() or
+// LSM_SHARED_GROUP (the blocks of every request must come from pump_blocks
+// with the same group).
 template <typename Real>
-hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s);
+hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, int group, hipStream_t s);
 }  // namespace qk

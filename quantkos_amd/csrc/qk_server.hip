@@ -390,7 +390,7 @@ hipError_t launch_pump_step(qk_server *srv, bool fp32) {
         ++a.n_req;
     }
     a.block_start[a.n_req] = blocks;
-    return launch_pump<Real>(a, 4, false, srv->pump_stream);   // LaguerreP03, the reference's basis
+    return launch_pump<Real>(a, 4, false, LSM_SHARED_GROUP, srv->pump_stream);   // LaguerreP03, the reference's basis
 }
 
 // Enqueues the merge of a request whose date-0 step was just queued.
@@ -702,8 +702,7 @@ extern "C" qk_status qk_server_submit(qk_server *s, const qk_contract *c, const 
     r->american = american;
     r->n_tiles = (c->n_paths + TILE - 1) / TILE;
     r->n_chunks = (r->n_tiles + MERGE_CHUNK - 1) / MERGE_CHUNK;
-    if (american)
-        r->nb = pump_blocks(c->n_paths, c->precision == QK_FP32 ? lsm_group<float>() : lsm_group<double>(), &r->chunk);
+    if (american) r->nb = pump_blocks(c->n_paths, LSM_SHARED_GROUP, &r->chunk);
     r->lay = layout_of(*c, american, r->n_tiles, r->n_chunks, r->nb);
     r->t_submit = clk::now();
     if (american && (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess ||
