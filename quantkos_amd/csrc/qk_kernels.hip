@@ -272,10 +272,18 @@ __device__ void lsm_solve(const double *s, double C[NB]) {
         if (count > 0.0) C[0] = s[S::b(0)] / count;
         return;
     }
+    // One thread runs this between two block barriers, so its latency is on
+    // the date's critical path: the divisions by the diagonal are products
+    // with its reciprocal (Math<double> sqrt and reciprocal, each within
+    // 0.5 ulp), which shortens the dependent chain.
     double Lm[NB][NB] = {};
+    double inv[NB];
+#pragma unroll
     for (int i = 0; i < NB; ++i) {
+#pragma unroll
         for (int j = 0; j <= i; ++j) {
             double acc = 0.0;
+#pragma unroll
             for (int k = 0; k < j; ++k) acc += Lm[i][k] * Lm[j][k];
             const double Mij = s[S::m(j, i)] + (i == j ? 1e-8 : 0.0);
             if (i == j) {
@@ -285,22 +293,27 @@ __device__ void lsm_solve(const double *s, double C[NB]) {
                     for (int k = 1; k < NB; ++k) C[k] = 0.0;
                     return;
                 }
-                Lm[i][i] = sqrt(val);
+                Lm[i][i] = Math<double>::sqrt(val);
+                inv[i] = Math<double>::rcp(Lm[i][i]);
             } else {
-                Lm[i][j] = (Mij - acc) / Lm[j][j];
+                Lm[i][j] = (Mij - acc) * inv[j];
             }
         }
     }
     double y[NB];
+#pragma unroll
     for (int i = 0; i < NB; ++i) {
         double acc = 0.0;
+#pragma unroll
         for (int k = 0; k < i; ++k) acc += Lm[i][k] * y[k];
-        y[i] = (s[S::b(i)] - acc) / Lm[i][i];
+        y[i] = (s[S::b(i)] - acc) * inv[i];
     }
+#pragma unroll
     for (int i = NB - 1; i >= 0; --i) {
         double acc = 0.0;
+#pragma unroll
         for (int k = i + 1; k < NB; ++k) acc += Lm[k][i] * C[k];
-        C[i] = (y[i] - acc) / Lm[i][i];
+        C[i] = (y[i] - acc) * inv[i];
     }
 }
 
@@ -331,16 +344,23 @@ __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpar
     double tot = 0.0;
 #pragma unroll
     for (int u = 0; u < UR; ++u) tot += acc[u];
-    if (active) red[lt] = tot;
+    // Lanes l and l + ST of a wavefront hold the same component: two (ST = 16)
+    // or one (ST = 32) exchange steps leave each component's wavefront sum in
+    // lanes 0 .. ST-1; one barrier; then lane k of wavefront 0 adds component
+    // k over the wavefronts in order and thread 0 gathers the NS sums.
+    const int lid = lt & 63, wv = lt >> 6;
+#pragma unroll
+    for (int o = ST; o < 64; o <<= 1) tot += __shfl_xor(tot, o, 64);
+    if (active && lid < ST) red[wv * ST + lid] = tot;
     __syncthreads();
-    for (int s = G / 2; s >= ST; s >>= 1) {
-        if (active && lt < s) red[lt] += red[lt + s];
-        __syncthreads();
-    }
-    if (active && lt == 0) {
+    if (active && wv == 0) {
+        double v = 0.0;
+        if (lid < ST)
+            for (int w = 0; w < G / 64; ++w) v += red[w * ST + lid];
         double sm[NS];
-        for (int k = 0; k < NS; ++k) sm[k] = red[k];
-        lsm_solve<NB>(sm, C);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) sm[k] = __shfl(v, k, 64);
+        if (lid == 0) lsm_solve<NB>(sm, C);
     }
 }
 
