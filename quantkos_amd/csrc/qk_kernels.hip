@@ -373,11 +373,11 @@ __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpar
 // writes the per-tile moments of the cash flows.  cfp[i - p0] is the cash flow
 // of path i (device memory, or LDS when CF_LDS).  Holds a block barrier when t >= 1: every
 // thread of the block must call it for the same t.
-template <typename Real, int NB, bool W, bool CF_LDS, int G>
+template <typename Real, int NB, bool W, bool CF_LDS, int G, typename Pre>
 __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, Real *cfp, int64_t n, int64_t p0,
                                                int64_t p1, int t, int N, int right, Real K, Real disc, Real itm_eps,
-                                               const double C[NB], double *part, double *psum, double *pm2,
-                                               uint32_t *pitm, double (*wpart)[LsmSums<NB>::NS]) {
+                                               double C[NB], double *part, double *psum, double *pm2,
+                                               uint32_t *pitm, double (*wpart)[LsmSums<NB>::NS], Pre &&pre) {
     using S = LsmSums<NB>;
     constexpr int NS = S::NS;
     const int lt = threadIdx.x & (G - 1), lid = lt & 63, wv = lt >> 6;
@@ -421,6 +421,9 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
         }
     };
     if (len > 0) fetch(0);
+    // The first pass's loads are in flight while pre(C) provides the date's
+    // coefficients (every thread calls it; it may hold block barriers).
+    pre(C);
     for (uint32_t r0 = 0; r0 < len; r0 += (uint32_t)G * U) {
         Real Sv[U], S1v[U], cfv[U];
         bool act[U];
@@ -551,24 +554,26 @@ __global__ __launch_bounds__(G) void k_lsm_step(PumpArgs<Real> a) {
     __shared__ double red[G];
     __shared__ double Cs[NB];
     __shared__ double wpart[G / 64][S::NS];
-    if (t < N - 1) {
-        double Cn[NB];
-        lsm_reduce_solve<NB, G>(q.bpart + (int64_t)(t & 1) * q.nb * S::STRIDE, q.nb, red, true, Cn);
-        if (threadIdx.x == 0)
-            for (int k = 0; k < NB; ++k) {
-                Cs[k] = Cn[k];
-                if (b == 0) q.coeffs[(int64_t)t * NB + k] = Cn[k];
-            }
-    }
-    __syncthreads();
-    double C[NB];
+    auto pre = [&](double C[NB]) {
+        if (t < N - 1) {
+            double Cn[NB];
+            lsm_reduce_solve<NB, G>(q.bpart + (int64_t)(t & 1) * q.nb * S::STRIDE, q.nb, red, true, Cn);
+            if (threadIdx.x == 0)
+                for (int k = 0; k < NB; ++k) {
+                    Cs[k] = Cn[k];
+                    if (b == 0) q.coeffs[(int64_t)t * NB + k] = Cn[k];
+                }
+        }
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? Cs[k] : 0.0;
+        for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? Cs[k] : 0.0;
+    };
+    double C[NB];
     const int64_t p0 = (int64_t)b * q.chunk;
     const int64_t p1 = p0 + q.chunk < q.n_paths ? p0 + q.chunk : q.n_paths;
     double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * S::STRIDE + (int64_t)b * S::STRIDE : nullptr;
     lsm_chunk_date<Real, NB, W, false, G>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
-                                part, q.psum, q.pm2, q.pitm, wpart);
+                                          part, q.psum, q.pm2, q.pitm, wpart, pre);
 }
 
 // Synchronisation of the cooperative backward pass (sync words, agent scope):
@@ -663,7 +668,7 @@ __global__ __launch_bounds__(lsm_group<Real>()) void k_lsm_coop(LsmCoopArgs<Real
         const int par = (t - 1) & 1;
         double *part = (t >= 1 && live) ? a.bpart + (int64_t)par * a.nb * S::STRIDE + (int64_t)vb * S::STRIDE : nullptr;
         lsm_chunk_date<Real, NB, W, true, G>(a.store, cfp, a.n_paths, p0, p1, t, N, a.right, a.K, a.disc, a.itm_eps, C,
-                                             part, a.psum, a.pm2, a.pitm, wpart);
+                                             part, a.psum, a.pm2, a.pitm, wpart, [](double *) {});
         if (t >= 1) {
             ++k_bar;
             if (coop_arrive(a.sync, k_bar)) {
