@@ -190,6 +190,26 @@ def test_american_matches_oracle(engine, oracle, precision, model, scheme):
     assert g.price >= 0.1 * 100.0 - 1e-9
 
 
+@pytest.mark.parametrize("precision,n_paths,n_steps", [(qk.FP64, 1, 6), (qk.FP64, 1025, 6), (qk.FP32, 4097, 6),
+                                                       (qk.FP64, 1_500_000, 12), (qk.FP32, 2_600_000, 8)])
+def test_american_chunk_edges_match_oracle(engine, oracle, precision, n_paths, n_steps):
+    """Sizes at the edges of the date kernel's partition: a single path, a
+    partial last row, and (1.5M FP64 / 2.6M FP32) chunks of more than one pass
+    of 8 rows per thread, whose second pass is loaded while the first is
+    computed, with a short last block.  Tolerances as in
+    test_american_matches_oracle."""
+    c = base_contract(qk.HESTON, qk.EULER, precision, n_paths=n_paths, n_steps=n_steps, price_scale=100.0)
+    g = engine.price_american(c, qk.PUT, 1.1)
+    o = oracle.price_american(omodel(oracle, c), c.n_paths, qk.PUT, 1.1, 100.0)
+    assert g.n == o.n == n_paths
+    if n_paths == 1:
+        assert math.isclose(g.price, o.price, rel_tol=1e-12, abs_tol=1e-12), (g.price, o.price)
+    elif precision == qk.FP64:
+        assert abs(g.price - o.price) < 0.02 * o.se, (g.price, o.price, o.se)
+    else:
+        assert abs(g.price - o.price) < 0.5 * o.se, (g.price, o.price, o.se)
+
+
 IKONEN_TOIVANEN = {8.0: 2.0, 9.0: 1.107641, 10.0: 0.520030, 11.0: 0.213668, 12.0: 0.082036}
 
 
