@@ -110,6 +110,11 @@ static __constant__ double EXP_P[12] = {0x1.0000000000000p+0, 0x1.0000000000000p
 __device__ constexpr double LN2_HI = 0x1.62e42ff000000p-1;
 __device__ constexpr double LN2_LO = -0x1.718432a1b0e26p-35;
 
+// Horner evaluation, lowest coefficient first.  The empty asm statement emits
+// no instruction; its "+s" operand makes the compiler hold each coefficient in
+// scalar registers at its use, so each step is one v_fma_f64 with a scalar
+// operand (without it the coefficients are kept in vector registers and each
+// step becomes a copy plus v_fmac_f64).
 template <int N>
 __device__ __forceinline__ double horner(const double (&c)[N], double x) {
     double p = c[N - 1];
