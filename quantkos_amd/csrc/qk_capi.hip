@@ -198,7 +198,7 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
         int64_t chunk = 0;
         // Gram partials: the smaller block size gives the larger block count.
         b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 +
-             (size_t)2 * pump_blocks(c.n_paths, LSM_SHARED_GROUP, &chunk) * LSM_STRIDE * 8;
+             (size_t)2 * pump_blocks(c.n_paths, LSM_SHARED_GROUP, PUMP_MAX_BLOCKS, &chunk) * LSM_STRIDE * 8;
     }
     return b;
 }
@@ -562,7 +562,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     const int N = c.n_steps;
     const int64_t n_tiles = (n + TILE - 1) / TILE;
     int64_t chunk = 0;
-    const int nb = pump_blocks(n, lsm_group<Real>(), &chunk);
+    const int nb = pump_blocks(n, lsm_group<Real>(), PUMP_MAX_BLOCKS, &chunk);
     if ((st = ctx->cf.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
     if ((st = ctx->coeffs.reserve(ctx, (size_t)N * n_basis * 8)) != QK_OK) return st;
     if ((st = ctx->bpart.reserve(ctx, (size_t)2 * nb * LSM_STRIDE * 8)) != QK_OK) return st;

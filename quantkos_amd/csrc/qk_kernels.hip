@@ -896,8 +896,8 @@ template hipError_t launch_select_hist<float>(const float *, int64_t, const Sele
 template hipError_t launch_select_hist<double>(const double *, int64_t, const SelectArgs &, uint32_t *, int,
                                                hipStream_t);
 
-int pump_blocks(int64_t n_paths, int group, int64_t *chunk) {
-    const int64_t nb = std::min<int64_t>((n_paths + group - 1) / group, PUMP_MAX_BLOCKS);
+int pump_blocks(int64_t n_paths, int group, int max_blocks, int64_t *chunk) {
+    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((n_paths + group - 1) / group, max_blocks));
     const int64_t c = ((n_paths + nb - 1) / nb + 255) / 256 * 256;
     *chunk = c;
     return (int)((n_paths + c - 1) / c);

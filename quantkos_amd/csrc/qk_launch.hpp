@@ -1,6 +1,9 @@
 // qk_launch.hpp -- kernel launchers (qk_kernels.hip) called by the C ABI layer.
 #pragma once
 
+#include <algorithm>
+#include <cmath>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -94,8 +97,17 @@ hipError_t launch_lsm_coop(const LsmCoopArgs<Real> &a, int n_basis, bool weighte
                            hipStream_t s);
 
 // Blocks and paths per block of one request's pump work (group = threads per
-// block, lsm_group<Real>()).
-int pump_blocks(int64_t n_paths, int group, int64_t *chunk);
+// block; at most max_blocks blocks, each a whole number of 256-path rows).
+int pump_blocks(int64_t n_paths, int group, int max_blocks, int64_t *chunk);
+// Block limit of a request in the server's pump: every block re-reads all of
+// its request's partials (nb^2 x 128 B per date), and the pump advances up to
+// PUMP_MAX_REQ requests per launch next to batch launches, so a request gets
+// about sqrt(n_paths / 32) blocks (config-4 burst rate +3 % over the limit of
+// 256 for every request).
+inline int shared_pump_max_blocks(int64_t n_paths) {
+    const int64_t s = (int64_t)std::ceil(std::sqrt((double)n_paths / 32.0));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(s, PUMP_MAX_BLOCKS));
+}
 
 // A request of a serving batch (one payoff, or a path store for LSM).
 constexpr int32_t REQ_PATHDEP = 1, REQ_STORE = 2;

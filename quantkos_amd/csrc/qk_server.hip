@@ -702,7 +702,8 @@ extern "C" qk_status qk_server_submit(qk_server *s, const qk_contract *c, const 
     r->american = american;
     r->n_tiles = (c->n_paths + TILE - 1) / TILE;
     r->n_chunks = (r->n_tiles + MERGE_CHUNK - 1) / MERGE_CHUNK;
-    if (american) r->nb = pump_blocks(c->n_paths, LSM_SHARED_GROUP, &r->chunk);
+    if (american)
+        r->nb = pump_blocks(c->n_paths, LSM_SHARED_GROUP, shared_pump_max_blocks(c->n_paths), &r->chunk);
     r->lay = layout_of(*c, american, r->n_tiles, r->n_chunks, r->nb);
     r->t_submit = clk::now();
     if (american && (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess ||
