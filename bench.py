@@ -115,11 +115,16 @@ def cpu_baseline(target_s=15.0):
             "sample": "%d paths x %d steps of the same QE workload (3 payoffs), FP64, %.1f s" % (n2, N_STEPS, el)}
 
 
+README_PUBLISHED_PRICE, README_PUBLISHED_SE = 5.110195, 0.003025
+README_PUBLISHED_S = {"fp32": 0.0976, "fp64": 4.9191}   # README.md:334, :336 (RTX 3060 Laptop)
+
+
 def other_configs(engine):
     """The other single-GPU workloads at their full sizes: Euler European FP64
     against the Heston characteristic-function price; Bates QE American put by
     Longstaff-Schwartz (HBM path store, per-date Gram reduction and solve); the
-    Greek stencil as lanes of one launch.  Each is timed once after a warm-up."""
+    Greek stencil as lanes of one launch; the reference's published README
+    benchmark.  Each is timed once after a warm-up."""
     import quantkos_amd as qk
     out = {}
     # Heston European call, Euler full truncation, 1,000,000 paths x 252 steps.
@@ -186,6 +191,30 @@ def other_configs(engine):
         "path_steps_per_s": lanes * c.n_paths * c.n_steps / wall,
         "european_call": {k: getattr(g[0], k) for k in names},
         "binary_cash_call": {k: getattr(g[1], k) for k in names}}
+    # The reference's one published benchmark (README.md:324-345,
+    # bench_asian_option_price_no_greeks.yaml): Heston QE arithmetic Asian call,
+    # 5,000,000 paths x 365 steps, published 5.110195 (SE 0.003025) and
+    # 0.0976 s (FP32 fast-math) / 4.9191 s (FP64) of "Total time spent for the
+    # computations" on an RTX 3060 Laptop GPU.  wall_ms is the whole pricing
+    # call (launch, merge, result copy), the counterpart of that line.
+    progress("README benchmark: QE Asian 5M x 365 (FP32, FP64)")
+    rd = {}
+    for prec, name, pub_s in ((qk.FP32, "fp32", README_PUBLISHED_S["fp32"]), (qk.FP64, "fp64", README_PUBLISHED_S["fp64"])):
+        c = qk.contract(model=qk.HESTON, scheme=qk.QE, precision=prec, n_steps=365, n_paths=5_000_000,
+                        seed=88471920573105, S0=1.0, v0=0.05, r=0.03, q=0.02, T=1.0, kappa=5.0, theta=0.05,
+                        sigma=0.5, rho=-0.8, price_scale=100.0)
+        p = qk.payoff(qk.ASIAN, qk.CALL, 1.0)
+        engine.price(c, p)
+        t0 = time.perf_counter()
+        st = engine.price(c, p)
+        wall = time.perf_counter() - t0
+        tm = engine.timing()
+        rd[name] = {"price": st.price, "se": st.se, "wall_ms": wall * 1e3, "kernel_ms": tm.path_kernel_ms,
+                    "path_steps_per_s": 5e6 * 365 / wall,
+                    "err_vs_published_in_combined_se": (st.price - README_PUBLISHED_PRICE) /
+                    math.sqrt(st.se ** 2 + README_PUBLISHED_SE ** 2),
+                    "published_rtx3060_s": pub_s, "speedup_vs_published_other_hardware": pub_s / wall}
+    out["readme_qe_asian_5Mx365"] = rd
     return out
 
 
