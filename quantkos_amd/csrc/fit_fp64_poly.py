@@ -3,13 +3,16 @@
 Each polynomial is the Chebyshev near-minimax fit (mpmath.chebyfit, 50 digits)
 of a smooth function on a reduced interval, rounded to the nearest double:
 
-  log:  P(z) = (2 atanh(s)/s - 2) / z,  z = s^2 in [0, (3 - 2 sqrt 2)^2]
-        (log(1+f) = 2 atanh(s), s = f/(2+f), f = m - 1, m in [sqrt(1/2), sqrt(2)));
-        7 coefficients, z P(z) absolute error 9e-18.
   sin:  S(w) = sin(pi/2 sqrt w) / sqrt w,  w = r^2 in [0, 1/4];  7 coefficients.
   cos:  C(w) = cos(pi/2 sqrt w),            w in [0, 1/4];       8 coefficients.
   exp:  E(r) = e^r on r in [-0.35, 0.35] (|r| <= ln2/2 after the reduction);
         12 coefficients, relative error 5e-18.
+  log1p for the table-driven log:  Q(r) = (log1p(r) - r) / r^2 on |r| <= R_LOG
+        (6 coefficients), and LOG_TAB: for m in [sqrt(1/2), sqrt(2)) split into 128
+        intervals by the high word of m (steps of 2^13 from 0x3fe6a09e), invc = 1/c
+        at the interval's centre c rounded to 7 significant bits (1 on the intervals
+        within R_LOG of 1), so r = m invc - 1 is exact in one fma, and -log(invc) as
+        a double plus a float remainder.
 
     python quantkos_amd/csrc/fit_fp64_poly.py      # prints the C initialisers
 
@@ -19,15 +22,7 @@ against these functions at 50 digits.
 import mpmath as mp
 
 mp.mp.dps = 50
-Z_MAX = (3 - 2 * mp.sqrt(2)) ** 2
 W_MAX = mp.mpf(1) / 4
-
-
-def log_p(z):
-    if z == 0:
-        return mp.mpf(2) / 3
-    s = mp.sqrt(z)
-    return (2 * mp.atanh(s) / s - 2) / z
 
 
 def sin_s(w):
@@ -41,9 +36,44 @@ def cos_c(w):
     return mp.cos(mp.pi / 2 * mp.sqrt(w))
 
 
+def log1p_q(r):
+    if abs(r) < mp.mpf(10) ** -20:
+        return -mp.mpf(1) / 2 + r / 3 - r ** 2 / 4
+    return (mp.log1p(r) - r) / r ** 2
+
+
+LOG_TAB_N, LOG_TAB_BASE, LOG_TAB_STEP = 128, 0x3fe6a09e, 1 << 13
+
+
+def _hi_word_double(h):
+    import struct
+    return struct.unpack("<d", struct.pack("<Q", h << 32))[0]
+
+
+def log_table():
+    """[(lhi, invc, llo)] per interval, and the largest |r| = |m invc - 1|."""
+    import numpy as np
+    out, rmax = [], mp.mpf(0)
+    for j in range(LOG_TAB_N):
+        lo = _hi_word_double(LOG_TAB_BASE + j * LOG_TAB_STEP)
+        hi = _hi_word_double(LOG_TAB_BASE + (j + 1) * LOG_TAB_STEP)
+        if max(abs(lo - 1.0), abs(hi - 1.0)) <= R_LOG:
+            invc = 1.0      # near 1 the result is log1p(m - 1) alone: no cancellation
+        else:
+            m, e = mp.frexp(2 / (mp.mpf(lo) + mp.mpf(hi)))
+            invc = float(mp.ldexp(mp.nint(m * 2 ** 7), e - 7))
+        rmax = max(rmax, abs(mp.mpf(lo) * invc - 1), abs(mp.mpf(hi) * invc - 1))
+        L = -mp.log(mp.mpf(invc))
+        lhi = float(L)
+        llo = float(np.float32(float(L - mp.mpf(lhi))))
+        out.append((lhi, invc, llo))
+    return out, rmax
+
+
+R_LOG = mp.mpf("0.009")
 R_EXP = mp.mpf("0.35")
-FITS = {"LOG_P": (log_p, 0, Z_MAX, 7), "SIN_S": (sin_s, 0, W_MAX, 7), "COS_C": (cos_c, 0, W_MAX, 8),
-        "EXP_P": (mp.exp, -R_EXP, R_EXP, 12)}
+FITS = {"SIN_S": (sin_s, 0, W_MAX, 7), "COS_C": (cos_c, 0, W_MAX, 8),
+        "EXP_P": (mp.exp, -R_EXP, R_EXP, 12), "LOG1P_Q": (log1p_q, -R_LOG, R_LOG, 6)}
 
 
 def coefficients(name):
@@ -57,6 +87,12 @@ def main():
     for name in FITS:
         cs = coefficients(name)
         print("%s[%d] = {%s};" % (name, len(cs), ", ".join(x.hex() for x in cs)))
+    tab, rmax = log_table()
+    print("// LOG_TAB: max |r| = %s" % mp.nstr(rmax, 6))
+    print("LOG_TAB[%d] = {" % len(tab))
+    for lhi, invc, llo in tab:
+        print("    {%s, %sf, %sf}," % (lhi.hex(), float(invc).hex(), float(llo).hex()))
+    print("};")
 
 
 if __name__ == "__main__":

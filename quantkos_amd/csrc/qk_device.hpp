@@ -46,8 +46,8 @@ __device__ __forceinline__ Words philox(uint32_t c0, uint32_t c1, uint32_t c2, u
 // the arguments the kernels give them (finite; positive for log, sqrt and the
 // reciprocal; u in [0, 1]), without the special-case handling of the library
 // routines.  Measured against long-double references over 2e7 arguments:
-// exp <= 0.9 ulp, log <= 0.85 ulp, sqrt and reciprocal <= 0.5 ulp, sin/cos
-// <= 1.5 ulp.
+// exp <= 0.9 ulp, log <= 1.5 ulp (99.97 % within 1 ulp), sqrt and reciprocal
+// <= 0.5 ulp, sin/cos <= 1.5 ulp.
 // ---------------------------------------------------------------------------
 template <typename Real> struct Math;
 
@@ -90,12 +90,8 @@ template <> struct Math<float> {
 
 // Polynomial coefficients, lowest order first; quantkos_amd/csrc/fit_fp64_poly.py
 // derives them (Chebyshev near-minimax fits at 50 digits, rounded to double).
-//   LOG_P: P(z) = (2 atanh(s)/s - 2)/z on z = s^2 in [0, (3 - 2 sqrt 2)^2]
 //   SIN_S: sin(pi/2 r)/r,  COS_C: cos(pi/2 r),  on w = r^2 in [0, 1/4]
 //   EXP_P: e^r on r in [-0.35, 0.35]
-static __constant__ double LOG_P[7] = {0x1.5555555555558p-1, 0x1.99999999952e2p-2, 0x1.2492492df148dp-2,
-                                        0x1.c71c62e5800a1p-3, 0x1.7462b4ab2ef6bp-3, 0x1.39fe606542ddep-3,
-                                        0x1.2b584aae78a57p-3};
 static __constant__ double SIN_S[7] = {0x1.921fb54442d18p+0, -0x1.4abbce625be41p-1, 0x1.466bc677587f8p-4,
                                         -0x1.32d2cce2e5b19p-8, 0x1.50782fda12d96p-13, -0x1.e30071afc3e59p-19,
                                         0x1.e3f38399551bfp-25};
@@ -106,6 +102,147 @@ static __constant__ double EXP_P[12] = {0x1.0000000000000p+0, 0x1.0000000000000p
                                         0x1.555555555555bp-3, 0x1.555555554e892p-5, 0x1.111111110ef9cp-7,
                                         0x1.6c16c189ae5d5p-10, 0x1.a01a01b24ec34p-13, 0x1.a0198d2d43afbp-16,
                                         0x1.71ddf2b060bb0p-19, 0x1.28b82404fa64fp-22, 0x1.af682e715258ap-26};
+// log1p(r) = r + r^2 LOG1P_Q(r) on |r| <= 0.009, and the table of the
+// table-driven log: for m in [sqrt(1/2), sqrt(2)) cut into 128 intervals by
+// the high word of m, {-log(invc) rounded to double, invc, the float
+// remainder of -log(invc)} with invc ~ 1 / (interval centre) of 7 significant
+// bits (1 near m = 1).
+static __constant__ double LOG1P_Q[6] = {-0x1.0000000000013p-1, 0x1.5555555555577p-2, -0x1.fffffff025cdcp-3,
+                                         0x1.9999998b824e9p-3, -0x1.555d4be9e7898p-3, 0x1.24995d37a01f0p-3};
+struct LogEntry {
+    double lhi;
+    float invc, llo;
+};
+static __constant__ LogEntry LOG_TAB[128] = {
+    {-0x1.5d1bdbf5809cap-2, 0x1.6800000000000p+0f, -0x1.4236380000000p-56f},
+    {-0x1.5d1bdbf5809cap-2, 0x1.6800000000000p+0f, -0x1.4236380000000p-56f},
+    {-0x1.51aad872df82dp-2, 0x1.6400000000000p+0f, -0x1.3927ac0000000p-59f},
+    {-0x1.51aad872df82dp-2, 0x1.6400000000000p+0f, -0x1.3927ac0000000p-59f},
+    {-0x1.4618bc21c5ec2p-2, 0x1.6000000000000p+0f, -0x1.f42dec0000000p-56f},
+    {-0x1.4618bc21c5ec2p-2, 0x1.6000000000000p+0f, -0x1.f42dec0000000p-56f},
+    {-0x1.3a64c556945eap-2, 0x1.5c00000000000p+0f, 0x1.c686520000000p-57f},
+    {-0x1.3a64c556945eap-2, 0x1.5c00000000000p+0f, 0x1.c686520000000p-57f},
+    {-0x1.2e8e2bae11d31p-2, 0x1.5800000000000p+0f, 0x1.8f4cdc0000000p-56f},
+    {-0x1.2e8e2bae11d31p-2, 0x1.5800000000000p+0f, 0x1.8f4cdc0000000p-56f},
+    {-0x1.2e8e2bae11d31p-2, 0x1.5800000000000p+0f, 0x1.8f4cdc0000000p-56f},
+    {-0x1.22941fbcf7966p-2, 0x1.5400000000000p+0f, 0x1.76f5ec0000000p-56f},
+    {-0x1.22941fbcf7966p-2, 0x1.5400000000000p+0f, 0x1.76f5ec0000000p-56f},
+    {-0x1.1675cababa60ep-2, 0x1.5000000000000p+0f, -0x1.ce63ea0000000p-61f},
+    {-0x1.1675cababa60ep-2, 0x1.5000000000000p+0f, -0x1.ce63ea0000000p-61f},
+    {-0x1.0a324e27390e3p-2, 0x1.4c00000000000p+0f, -0x1.7dcfde0000000p-56f},
+    {-0x1.0a324e27390e3p-2, 0x1.4c00000000000p+0f, -0x1.7dcfde0000000p-56f},
+    {-0x1.0a324e27390e3p-2, 0x1.4c00000000000p+0f, -0x1.7dcfde0000000p-56f},
+    {-0x1.fb9186d5e3e2bp-3, 0x1.4800000000000p+0f, 0x1.caaae60000000p-57f},
+    {-0x1.fb9186d5e3e2bp-3, 0x1.4800000000000p+0f, 0x1.caaae60000000p-57f},
+    {-0x1.e27076e2af2e6p-3, 0x1.4400000000000p+0f, 0x1.6157800000000p-59f},
+    {-0x1.e27076e2af2e6p-3, 0x1.4400000000000p+0f, 0x1.6157800000000p-59f},
+    {-0x1.e27076e2af2e6p-3, 0x1.4400000000000p+0f, 0x1.6157800000000p-59f},
+    {-0x1.c8ff7c79a9a22p-3, 0x1.4000000000000p+0f, 0x1.4f68a00000000p-57f},
+    {-0x1.c8ff7c79a9a22p-3, 0x1.4000000000000p+0f, 0x1.4f68a00000000p-57f},
+    {-0x1.af3c94e80bff3p-3, 0x1.3c00000000000p+0f, 0x1.398d000000000p-58f},
+    {-0x1.af3c94e80bff3p-3, 0x1.3c00000000000p+0f, 0x1.398d000000000p-58f},
+    {-0x1.af3c94e80bff3p-3, 0x1.3c00000000000p+0f, 0x1.398d000000000p-58f},
+    {-0x1.9525a9cf456b4p-3, 0x1.3800000000000p+0f, -0x1.d904c20000000p-57f},
+    {-0x1.9525a9cf456b4p-3, 0x1.3800000000000p+0f, -0x1.d904c20000000p-57f},
+    {-0x1.7ab890210d909p-3, 0x1.3400000000000p+0f, -0x1.be36b20000000p-59f},
+    {-0x1.7ab890210d909p-3, 0x1.3400000000000p+0f, -0x1.be36b20000000p-59f},
+    {-0x1.7ab890210d909p-3, 0x1.3400000000000p+0f, -0x1.be36b20000000p-59f},
+    {-0x1.5ff3070a793d4p-3, 0x1.3000000000000p+0f, 0x1.bc60f00000000p-58f},
+    {-0x1.5ff3070a793d4p-3, 0x1.3000000000000p+0f, 0x1.bc60f00000000p-58f},
+    {-0x1.5ff3070a793d4p-3, 0x1.3000000000000p+0f, 0x1.bc60f00000000p-58f},
+    {-0x1.44d2b6ccb7d1ep-3, 0x1.2c00000000000p+0f, -0x1.9f4f660000000p-57f},
+    {-0x1.44d2b6ccb7d1ep-3, 0x1.2c00000000000p+0f, -0x1.9f4f660000000p-57f},
+    {-0x1.44d2b6ccb7d1ep-3, 0x1.2c00000000000p+0f, -0x1.9f4f660000000p-57f},
+    {-0x1.29552f81ff523p-3, 0x1.2800000000000p+0f, -0x1.3017720000000p-57f},
+    {-0x1.29552f81ff523p-3, 0x1.2800000000000p+0f, -0x1.3017720000000p-57f},
+    {-0x1.29552f81ff523p-3, 0x1.2800000000000p+0f, -0x1.3017720000000p-57f},
+    {-0x1.0d77e7cd08e59p-3, 0x1.2400000000000p+0f, -0x1.9a5dc60000000p-57f},
+    {-0x1.0d77e7cd08e59p-3, 0x1.2400000000000p+0f, -0x1.9a5dc60000000p-57f},
+    {-0x1.0d77e7cd08e59p-3, 0x1.2400000000000p+0f, -0x1.9a5dc60000000p-57f},
+    {-0x1.e27076e2af2e6p-4, 0x1.2000000000000p+0f, 0x1.6157800000000p-60f},
+    {-0x1.e27076e2af2e6p-4, 0x1.2000000000000p+0f, 0x1.6157800000000p-60f},
+    {-0x1.e27076e2af2e6p-4, 0x1.2000000000000p+0f, 0x1.6157800000000p-60f},
+    {-0x1.a926d3a4ad563p-4, 0x1.1c00000000000p+0f, -0x1.942f480000000p-58f},
+    {-0x1.a926d3a4ad563p-4, 0x1.1c00000000000p+0f, -0x1.942f480000000p-58f},
+    {-0x1.a926d3a4ad563p-4, 0x1.1c00000000000p+0f, -0x1.942f480000000p-58f},
+    {-0x1.6f0d28ae56b4cp-4, 0x1.1800000000000p+0f, 0x1.906d9a0000000p-58f},
+    {-0x1.6f0d28ae56b4cp-4, 0x1.1800000000000p+0f, 0x1.906d9a0000000p-58f},
+    {-0x1.6f0d28ae56b4cp-4, 0x1.1800000000000p+0f, 0x1.906d9a0000000p-58f},
+    {-0x1.6f0d28ae56b4cp-4, 0x1.1800000000000p+0f, 0x1.906d9a0000000p-58f},
+    {-0x1.341d7961bd1d1p-4, 0x1.1400000000000p+0f, 0x1.b599f20000000p-58f},
+    {-0x1.341d7961bd1d1p-4, 0x1.1400000000000p+0f, 0x1.b599f20000000p-58f},
+    {-0x1.341d7961bd1d1p-4, 0x1.1400000000000p+0f, 0x1.b599f20000000p-58f},
+    {-0x1.f0a30c01162a6p-5, 0x1.1000000000000p+0f, -0x1.85f3260000000p-59f},
+    {-0x1.f0a30c01162a6p-5, 0x1.1000000000000p+0f, -0x1.85f3260000000p-59f},
+    {-0x1.f0a30c01162a6p-5, 0x1.1000000000000p+0f, -0x1.85f3260000000p-59f},
+    {-0x1.f0a30c01162a6p-5, 0x1.1000000000000p+0f, -0x1.85f3260000000p-59f},
+    {-0x1.77458f632dcfcp-5, 0x1.0c00000000000p+0f, -0x1.18d3ca0000000p-59f},
+    {-0x1.77458f632dcfcp-5, 0x1.0c00000000000p+0f, -0x1.18d3ca0000000p-59f},
+    {-0x1.77458f632dcfcp-5, 0x1.0c00000000000p+0f, -0x1.18d3ca0000000p-59f},
+    {-0x1.f829b0e783300p-6, 0x1.0800000000000p+0f, -0x1.33e3f00000000p-60f},
+    {-0x1.f829b0e783300p-6, 0x1.0800000000000p+0f, -0x1.33e3f00000000p-60f},
+    {-0x1.f829b0e783300p-6, 0x1.0800000000000p+0f, -0x1.33e3f00000000p-60f},
+    {-0x1.f829b0e783300p-6, 0x1.0800000000000p+0f, -0x1.33e3f00000000p-60f},
+    {-0x1.fc0a8b0fc03e4p-7, 0x1.0400000000000p+0f, 0x1.83092c0000000p-62f},
+    {-0x1.fc0a8b0fc03e4p-7, 0x1.0400000000000p+0f, 0x1.83092c0000000p-62f},
+    {-0x1.fc0a8b0fc03e4p-7, 0x1.0400000000000p+0f, 0x1.83092c0000000p-62f},
+    {-0x1.fc0a8b0fc03e4p-7, 0x1.0400000000000p+0f, 0x1.83092c0000000p-62f},
+    {0x0.0p+0, 0x1.0000000000000p+0f, 0x0.0p+0f},
+    {0x0.0p+0, 0x1.0000000000000p+0f, 0x0.0p+0f},
+    {0x0.0p+0, 0x1.0000000000000p+0f, 0x0.0p+0f},
+    {0x1.0205658935847p-6, 0x1.f800000000000p-1f, 0x1.27c8e80000000p-60f},
+    {0x1.0205658935847p-6, 0x1.f800000000000p-1f, 0x1.27c8e80000000p-60f},
+    {0x1.8492528c8cabfp-6, 0x1.f400000000000p-1f, -0x1.d192d00000000p-60f},
+    {0x1.0415d89e74444p-5, 0x1.f000000000000p-1f, 0x1.c05cf20000000p-59f},
+    {0x1.466aed42de3eap-5, 0x1.ec00000000000p-1f, -0x1.cdd6f80000000p-59f},
+    {0x1.894aa149fb343p-5, 0x1.e800000000000p-1f, 0x1.a8be980000000p-60f},
+    {0x1.ccb73cdddb2ccp-5, 0x1.e400000000000p-1f, -0x1.e48fb00000000p-59f},
+    {0x1.08598b59e3a07p-4, 0x1.e000000000000p-1f, -0x1.dd700a0000000p-58f},
+    {0x1.2aa04a44717a5p-4, 0x1.dc00000000000p-1f, -0x1.d15d380000000p-58f},
+    {0x1.4d3115d207eacp-4, 0x1.d800000000000p-1f, 0x1.769f420000000p-58f},
+    {0x1.700d30aeac0e1p-4, 0x1.d400000000000p-1f, -0x1.7256620000000p-61f},
+    {0x1.700d30aeac0e1p-4, 0x1.d400000000000p-1f, -0x1.7256620000000p-61f},
+    {0x1.9335e5d594989p-4, 0x1.d000000000000p-1f, -0x1.478a860000000p-58f},
+    {0x1.b6ac88dad5b1cp-4, 0x1.cc00000000000p-1f, -0x1.0057ee0000000p-59f},
+    {0x1.da727638446a2p-4, 0x1.c800000000000p-1f, 0x1.401fa80000000p-58f},
+    {0x1.fe89139dbd566p-4, 0x1.c400000000000p-1f, -0x1.ac9f420000000p-58f},
+    {0x1.fe89139dbd566p-4, 0x1.c400000000000p-1f, -0x1.ac9f420000000p-58f},
+    {0x1.1178e8227e47cp-3, 0x1.c000000000000p-1f, -0x1.0e63a60000000p-58f},
+    {0x1.23d712a49c202p-3, 0x1.bc00000000000p-1f, -0x1.6e38160000000p-57f},
+    {0x1.365fcb0159016p-3, 0x1.b800000000000p-1f, 0x1.7d411a0000000p-58f},
+    {0x1.365fcb0159016p-3, 0x1.b800000000000p-1f, 0x1.7d411a0000000p-58f},
+    {0x1.4913d8333b561p-3, 0x1.b400000000000p-1f, -0x1.0d56040000000p-58f},
+    {0x1.5bf406b543db2p-3, 0x1.b000000000000p-1f, -0x1.1f5b440000000p-61f},
+    {0x1.6f0128b756abcp-3, 0x1.ac00000000000p-1f, -0x1.8de59c0000000p-57f},
+    {0x1.6f0128b756abcp-3, 0x1.ac00000000000p-1f, -0x1.8de59c0000000p-57f},
+    {0x1.823c16551a3c2p-3, 0x1.a800000000000p-1f, -0x1.1232ce0000000p-57f},
+    {0x1.95a5adcf7017fp-3, 0x1.a400000000000p-1f, 0x1.142c500000000p-58f},
+    {0x1.95a5adcf7017fp-3, 0x1.a400000000000p-1f, 0x1.142c500000000p-58f},
+    {0x1.a93ed3c8ad9e3p-3, 0x1.a000000000000p-1f, 0x1.bcafaa0000000p-57f},
+    {0x1.bd087383bd8adp-3, 0x1.9c00000000000p-1f, 0x1.dd35600000000p-60f},
+    {0x1.bd087383bd8adp-3, 0x1.9c00000000000p-1f, 0x1.dd35600000000p-60f},
+    {0x1.d1037f2655e7bp-3, 0x1.9800000000000p-1f, 0x1.6062920000000p-57f},
+    {0x1.e530effe71012p-3, 0x1.9400000000000p-1f, 0x1.2276040000000p-59f},
+    {0x1.e530effe71012p-3, 0x1.9400000000000p-1f, 0x1.2276040000000p-59f},
+    {0x1.f991c6cb3b379p-3, 0x1.9000000000000p-1f, 0x1.f665060000000p-57f},
+    {0x1.f991c6cb3b379p-3, 0x1.9000000000000p-1f, 0x1.f665060000000p-57f},
+    {0x1.07138604d5862p-2, 0x1.8c00000000000p-1f, 0x1.cdb16e0000000p-56f},
+    {0x1.1178e8227e47cp-2, 0x1.8800000000000p-1f, -0x1.0e63a60000000p-57f},
+    {0x1.1178e8227e47cp-2, 0x1.8800000000000p-1f, -0x1.0e63a60000000p-57f},
+    {0x1.1bf99635a6b95p-2, 0x1.8400000000000p-1f, -0x1.12aeb80000000p-57f},
+    {0x1.1bf99635a6b95p-2, 0x1.8400000000000p-1f, -0x1.12aeb80000000p-57f},
+    {0x1.269621134db92p-2, 0x1.8000000000000p-1f, 0x1.e0efae0000000p-56f},
+    {0x1.269621134db92p-2, 0x1.8000000000000p-1f, 0x1.e0efae0000000p-56f},
+    {0x1.314f1e1d35ce4p-2, 0x1.7c00000000000p-1f, -0x1.3d69900000000p-56f},
+    {0x1.3c25277333184p-2, 0x1.7800000000000p-1f, -0x1.2ad27e0000000p-56f},
+    {0x1.3c25277333184p-2, 0x1.7800000000000p-1f, -0x1.2ad27e0000000p-56f},
+    {0x1.4718dc271c41bp-2, 0x1.7400000000000p-1f, 0x1.8fb4c20000000p-60f},
+    {0x1.4718dc271c41bp-2, 0x1.7400000000000p-1f, 0x1.8fb4c20000000p-60f},
+    {0x1.522ae0738a3d8p-2, 0x1.7000000000000p-1f, -0x1.8f7e9c0000000p-57f},
+    {0x1.522ae0738a3d8p-2, 0x1.7000000000000p-1f, -0x1.8f7e9c0000000p-57f},
+    {0x1.5d5bddf595f30p-2, 0x1.6c00000000000p-1f, -0x1.6541140000000p-56f},
+    {0x1.5d5bddf595f30p-2, 0x1.6c00000000000p-1f, -0x1.6541140000000p-56f},
+};
 // ln 2 = LN2_HI + LN2_LO; LN2_HI has 32 significant bits, so k * LN2_HI is exact.
 __device__ constexpr double LN2_HI = 0x1.62e42ff000000p-1;
 __device__ constexpr double LN2_LO = -0x1.718432a1b0e26p-35;
@@ -137,13 +274,14 @@ template <> struct Math<double> {
         r = __builtin_fma(-n, LN2_LO, r);
         return __builtin_amdgcn_ldexp(horner(EXP_P, r), (int)n);
     }
-    // Natural log of a positive normal finite x, by the reduction of Sun's
-    // fdlibm e_log.c: x = 2^k m, m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact),
-    // s = f/(2+f), log(1+f) = 2 atanh(s) = f - (h - s (h + z P(z))), h = f^2/2,
-    // z = s^2.  k and m come from the high word: subtracting the high word of
-    // sqrt(1/2) puts k in its exponent field, and removing k from the exponent
-    // of x leaves m.  s is f times v_rcp_f64 (relative error ~2^-23) refined by
-    // two residual steps; the result is k LN2_HI + (f - (h - (s (h + R) + k LN2_LO))).
+    // Natural log of a positive normal finite x, table-driven (the scheme of
+    // Tang, ACM TOMS 16 (1990), with an inexact reciprocal table): x = 2^k m,
+    // m in [sqrt(1/2), sqrt(2)), k and m from the high word (subtracting the
+    // high word of sqrt(1/2) puts k in its exponent field, and removing k
+    // from the exponent of x leaves m); the next 7 bits pick a table entry;
+    // r = m invc - 1 is exact in one fma (invc has 7 significant bits) and
+    // log x = k ln2 - log(invc) + log1p(r), the two constant parts each split
+    // into a high and a low term.
     __device__ static __forceinline__ double log(double x) {
         const uint64_t ix = __builtin_bit_cast(uint64_t, x);
         const uint32_t hx = (uint32_t)(ix >> 32);
@@ -151,18 +289,13 @@ template <> struct Math<double> {
         const int k = tmp >> 20;
         const uint32_t hm = hx - ((uint32_t)tmp & 0xfff00000u);
         const double m = __builtin_bit_cast(double, ((uint64_t)hm << 32) | (uint32_t)ix);
-        const double f = m - 1.0;
-        const double d = 2.0 + f;
-        const double r0 = __builtin_amdgcn_rcp(d);
-        double s = f * r0;
-        s = __builtin_fma(__builtin_fma(-d, s, f), r0, s);
-        s = __builtin_fma(__builtin_fma(-d, s, f), r0, s);
-        const double z = s * s;
-        const double R = z * horner(LOG_P, z);
-        const double h = (0.5 * f) * f;
+        const LogEntry e = LOG_TAB[((uint32_t)tmp >> 13) & 127u];
+        const double r = __builtin_fma(m, (double)e.invc, -1.0);
+        const double lp = __builtin_fma(r * r, horner(LOG1P_Q, r), r);
         const double kd = (double)k;
-        const double t = __builtin_fma(s, h + R, kd * LN2_LO);
-        return __builtin_fma(kd, LN2_HI, f - (h - t));
+        const double hi = __builtin_fma(kd, LN2_HI, e.lhi);
+        const double lo = __builtin_fma(kd, LN2_LO, (double)e.llo);
+        return hi + (lp + lo);
     }
     // sqrt of a finite x >= 0: one Goldschmidt step from v_rsq_f64 (g ~ sqrt x,
     // h ~ 1/(2 sqrt x)), then the Newton residual step g + h (x - g^2).  The max

@@ -1,14 +1,17 @@
 """The FP64 device polynomials (qk_device.hpp, Math<double>) against the
 functions they approximate, at 50 digits.
 
-The coefficients are read from the header as written, so a mistyped digit, a
-dropped term or a reversed order fails here.  The bounds are the near-minimax
-fit error plus the rounding of the coefficients to double:
-  log:  z P(z) vs 2 atanh(s)/s - 2 on z in [0, (3 - 2 sqrt 2)^2]    < 2e-17 absolute
-  sin:  S(w)   vs sin(pi/2 sqrt w)/sqrt w on w in [0, 1/4]          < 2.5e-16 absolute
-  cos:  C(w)   vs cos(pi/2 sqrt w)                                  < 6e-17 absolute
-  exp:  E(r)   vs e^r on r in [-0.35, 0.35]                         < 1e-17 relative
-and the split of ln 2 into LN2_HI (32 significant bits) + LN2_LO.
+The coefficients and the log table are read from the header as written, so a
+mistyped digit, a dropped term or a reversed order fails here.  The bounds are
+the near-minimax fit error plus the rounding of the coefficients to double:
+  log1p: r + r^2 Q(r) vs log1p(r) on |r| <= 0.009                   < 2e-17 relative
+  sin:   S(w) vs sin(pi/2 sqrt w)/sqrt w on w in [0, 1/4]           < 2.5e-16 absolute
+  cos:   C(w) vs cos(pi/2 sqrt w)                                   < 6e-17 absolute
+  exp:   E(r) vs e^r on r in [-0.35, 0.35]                          < 1e-17 relative
+the log table (each entry's interval of m gives |m invc - 1| <= 0.009, invc
+has at most 7 significant bits so that m invc - 1 is exact in one fma, and
+lhi + llo = -log(invc) to 2^-76), and the split of ln 2 into LN2_HI (32
+significant bits) + LN2_LO.
 """
 import os
 import re
@@ -47,16 +50,39 @@ def grid(hi, n=1500):
     return [hi * mp.mpf(i) / n for i in range(n + 1)]
 
 
-def test_log_polynomial():
+def test_log1p_polynomial():
     mp.mp.dps = 50
-    c = header_array("LOG_P")
-    zmax = (3 - 2 * mp.sqrt(2)) ** 2
+    c = header_array("LOG1P_Q")
     worst = 0
-    for z in grid(zmax)[1:]:
-        s = mp.sqrt(z)
-        exact = 2 * mp.atanh(s) / s - 2
-        worst = max(worst, abs(z * poly(c, z) - exact))
+    for i in range(-1500, 1501):
+        if i == 0:
+            continue
+        r = mp.mpf("0.009") * i / 1500
+        worst = max(worst, abs((r + r * r * poly(c, r)) / mp.log1p(r) - 1))
     assert worst < 2e-17, worst
+
+
+def header_log_table():
+    text = open(HEADER).read()
+    m = re.search(r"\bLOG_TAB\[(\d+)\]\s*=\s*\{(.*?)\n\};", text, re.S)
+    assert m
+    rows = re.findall(r"\{\s*([^,}]+),\s*([^,}]+?)f,\s*([^,}]+?)f\s*\}", m.group(2))
+    assert len(rows) == int(m.group(1)) == 128
+    return [tuple(float.fromhex(x.strip()) for x in row) for row in rows]
+
+
+def test_log_table():
+    import struct
+    mp.mp.dps = 50
+    hi_word = lambda h: struct.unpack("<d", struct.pack("<Q", h << 32))[0]
+    for j, (lhi, invc, llo) in enumerate(header_log_table()):
+        mant, _ = mp.frexp(mp.mpf(invc))
+        assert mant * 2 ** 7 == int(mant * 2 ** 7), (j, invc)           # <= 7 significant bits
+        assert abs(mp.mpf(lhi) + mp.mpf(llo) + mp.log(invc)) < mp.mpf(2) ** -76, j
+        lo, hi = hi_word(0x3fe6a09e + j * 2 ** 13), hi_word(0x3fe6a09e + (j + 1) * 2 ** 13)
+        assert max(abs(mp.mpf(lo) * invc - 1), abs(mp.mpf(hi) * invc - 1)) <= mp.mpf("0.009"), j
+        if lo <= 1.0 < hi:
+            assert invc == 1.0 and lhi == 0.0 and llo == 0.0
 
 
 def test_sin_cos_polynomials():
@@ -95,5 +121,7 @@ def test_fit_script_reproduces_header():
     spec = importlib.util.spec_from_file_location("fit_fp64_poly", FIT)
     fit = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(fit)
-    for name in ("LOG_P", "SIN_S", "COS_C", "EXP_P"):
+    for name in ("SIN_S", "COS_C", "EXP_P", "LOG1P_Q"):
         assert fit.coefficients(name) == header_array(name), name
+    tab, _ = fit.log_table()
+    assert [(a, float(b), c) for a, b, c in tab] == header_log_table()
