@@ -3,8 +3,11 @@
 Each polynomial is the Chebyshev near-minimax fit (mpmath.chebyfit, 50 digits)
 of a smooth function on a reduced interval, rounded to the nearest double:
 
-  sin:  S(w) = sin(pi/2 sqrt w) / sqrt w,  w = r^2 in [0, 1/4];  7 coefficients.
-  cos:  C(w) = cos(pi/2 sqrt w),            w in [0, 1/4];       8 coefficients.
+  sin/cos(2 pi u) = table entry j (angle 2 pi j/64) combined with the angle
+        2 pi r/64, 64 u = j + r, |r| <= 1/2:
+        SIN64_S(w) = sin(pi/32 sqrt w) / sqrt w,       w = r^2 in [0, 1/4];  4 coefficients,
+        COS64_M(w) = (cos(pi/32 sqrt w) - 1) / w;                              4 coefficients,
+        SINCOS_TAB[j] = {sin, cos}(2 pi j / 64) as double + double remainder.
   exp:  E(r) = e^r on r in [-0.35, 0.35] (|r| <= ln2/2 after the reduction);
         12 coefficients, relative error 5e-18.
   log1p for the table-driven log:  Q(r) = (log1p(r) - r) / r^2 on |r| <= R_LOG
@@ -25,15 +28,27 @@ mp.mp.dps = 50
 W_MAX = mp.mpf(1) / 4
 
 
-def sin_s(w):
+def sin64_s(w):
     if w == 0:
-        return mp.pi / 2
+        return mp.pi / 32
     t = mp.sqrt(w)
-    return mp.sin(mp.pi / 2 * t) / t
+    return mp.sin(mp.pi / 32 * t) / t
 
 
-def cos_c(w):
-    return mp.cos(mp.pi / 2 * mp.sqrt(w))
+def cos64_m(w):
+    if w == 0:
+        return -(mp.pi / 32) ** 2 / 2
+    return (mp.cos(mp.pi / 32 * mp.sqrt(w)) - 1) / w
+
+
+def sincos_table():
+    """[(sin_hi, cos_hi, sin_lo, cos_lo)] of the angles 2 pi j / 64, j = 0..63."""
+    out = []
+    for j in range(64):
+        sv, cv = mp.sin(2 * mp.pi * j / 64), mp.cos(2 * mp.pi * j / 64)
+        sh, ch = float(sv), float(cv)
+        out.append((sh, ch, float(sv - sh), float(cv - ch)))
+    return out
 
 
 def log1p_q(r):
@@ -72,7 +87,7 @@ def log_table():
 
 R_LOG = mp.mpf("0.009")
 R_EXP = mp.mpf("0.35")
-FITS = {"SIN_S": (sin_s, 0, W_MAX, 7), "COS_C": (cos_c, 0, W_MAX, 8),
+FITS = {"SIN64_S": (sin64_s, 0, W_MAX, 4), "COS64_M": (cos64_m, 0, W_MAX, 4),
         "EXP_P": (mp.exp, -R_EXP, R_EXP, 12), "LOG1P_Q": (log1p_q, -R_LOG, R_LOG, 6)}
 
 
@@ -87,6 +102,10 @@ def main():
     for name in FITS:
         cs = coefficients(name)
         print("%s[%d] = {%s};" % (name, len(cs), ", ".join(x.hex() for x in cs)))
+    print("SINCOS_TAB[64] = {")
+    for e in sincos_table():
+        print("    {%s}," % ", ".join(x.hex() for x in e))
+    print("};")
     tab, rmax = log_table()
     print("// LOG_TAB: max |r| = %s" % mp.nstr(rmax, 6))
     print("LOG_TAB[%d] = {" % len(tab))

@@ -47,7 +47,8 @@ __device__ __forceinline__ Words philox(uint32_t c0, uint32_t c1, uint32_t c2, u
 // reciprocal; u in [0, 1]), without the special-case handling of the library
 // routines.  Measured against long-double references over 2e7 arguments:
 // exp <= 0.9 ulp, log <= 1.5 ulp (99.97 % within 1 ulp), sqrt and reciprocal
-// <= 0.5 ulp, sin/cos <= 1.5 ulp.
+// <= 0.5 ulp, sin/cos(2 pi u) <= 2.2 ulp where |value| > 0.01 and within
+// 7.1e-17 everywhere.
 // ---------------------------------------------------------------------------
 template <typename Real> struct Math;
 
@@ -90,14 +91,83 @@ template <> struct Math<float> {
 
 // Polynomial coefficients, lowest order first; quantkos_amd/csrc/fit_fp64_poly.py
 // derives them (Chebyshev near-minimax fits at 50 digits, rounded to double).
-//   SIN_S: sin(pi/2 r)/r,  COS_C: cos(pi/2 r),  on w = r^2 in [0, 1/4]
 //   EXP_P: e^r on r in [-0.35, 0.35]
-static __constant__ double SIN_S[7] = {0x1.921fb54442d18p+0, -0x1.4abbce625be41p-1, 0x1.466bc677587f8p-4,
-                                        -0x1.32d2cce2e5b19p-8, 0x1.50782fda12d96p-13, -0x1.e30071afc3e59p-19,
-                                        0x1.e3f38399551bfp-25};
-static __constant__ double COS_C[8] = {0x1.0000000000000p+0, -0x1.3bd3cc9be45dep+0, 0x1.03c1f081b5aacp-2,
-                                        -0x1.55d3c7e3c90f8p-6, 0x1.e1f5068355e15p-11, -0x1.a6d1ec7906c20p-16,
-                                        0x1.f9cc41140bb60p-22, -0x1.b264ba152378ap-28};
+// sin/cos(2 pi u) from the table of the 64 angles 2 pi j/64 (each as a double
+// and a double remainder) and the angle 2 pi r/64, |r| <= 1/2:
+// SIN64_S(w) = sin(pi/32 sqrt w)/sqrt w, COS64_M(w) = (cos(pi/32 sqrt w) - 1)/w.
+static __constant__ double SIN64_S[4] = {0x1.921fb54442d18p-4, -0x1.4abbce625bd02p-13,
+                                        0x1.466bc66a361e0p-24, -0x1.32cd8b0fba547p-36};
+static __constant__ double COS64_M[4] = {-0x1.3bd3cc9be45dep-8, 0x1.03c1f081b5a5ap-18,
+                                        -0x1.55d3c7db89f7bp-30, 0x1.e1ee6b4c2f23fp-43};
+struct SinCosEntry {
+    double s, c, s_lo, c_lo;
+};
+static __constant__ SinCosEntry SINCOS_TAB[64] = {
+    {0x0.0p+0, 0x1.0000000000000p+0, 0x0.0p+0, 0x0.0p+0},
+    {0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1, -0x1.e2718d26ed688p-60, -0x1.87df6378811c7p-55},
+    {0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1, -0x1.26d19b9ff8d82p-57, 0x1.562172a361fd3p-56},
+    {0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1, -0x1.5d28da2c4612dp-56, 0x1.760b1e2e3f81ep-55},
+    {0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1, -0x1.72cedd3d5a610p-57, 0x1.457e610231ac2p-56},
+    {0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1, 0x1.e0d891d3c6841p-58, -0x1.6e0b1757c8d07p-56},
+    {0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1, 0x1.b25dd267f6600p-55, 0x1.9f630e8b6dac8p-60},
+    {0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1, 0x1.8076a2cfdc6b3p-57, -0x1.2c5e12ed1336dp-55},
+    {0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1, -0x1.bdd3413b26456p-55, -0x1.bdd3413b26456p-55},
+    {0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1, -0x1.2c5e12ed1336dp-55, 0x1.8076a2cfdc6b3p-57},
+    {0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1, 0x1.9f630e8b6dac8p-60, 0x1.b25dd267f6600p-55},
+    {0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2, -0x1.6e0b1757c8d07p-56, 0x1.e0d891d3c6841p-58},
+    {0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2, 0x1.457e610231ac2p-56, -0x1.72cedd3d5a610p-57},
+    {0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2, 0x1.760b1e2e3f81ep-55, -0x1.5d28da2c4612dp-56},
+    {0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3, 0x1.562172a361fd3p-56, -0x1.26d19b9ff8d82p-57},
+    {0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4, -0x1.87df6378811c7p-55, -0x1.e2718d26ed688p-60},
+    {0x1.0000000000000p+0, -0x1.8338bfdf4415ap-171, 0x0.0p+0, 0x1.d89cd9128a504p-226},
+    {0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4, -0x1.87df6378811c7p-55, 0x1.e2718d26ed688p-60},
+    {0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3, 0x1.562172a361fd3p-56, 0x1.26d19b9ff8d82p-57},
+    {0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2, 0x1.760b1e2e3f81ep-55, 0x1.5d28da2c4612dp-56},
+    {0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2, 0x1.457e610231ac2p-56, 0x1.72cedd3d5a610p-57},
+    {0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2, -0x1.6e0b1757c8d07p-56, -0x1.e0d891d3c6841p-58},
+    {0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1, 0x1.9f630e8b6dac8p-60, -0x1.b25dd267f6600p-55},
+    {0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1, -0x1.2c5e12ed1336dp-55, -0x1.8076a2cfdc6b3p-57},
+    {0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1, -0x1.bdd3413b26456p-55, 0x1.bdd3413b26456p-55},
+    {0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1, 0x1.8076a2cfdc6b3p-57, 0x1.2c5e12ed1336dp-55},
+    {0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1, 0x1.b25dd267f6600p-55, -0x1.9f630e8b6dac8p-60},
+    {0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1, 0x1.e0d891d3c6841p-58, 0x1.6e0b1757c8d07p-56},
+    {0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1, -0x1.72cedd3d5a610p-57, -0x1.457e610231ac2p-56},
+    {0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1, -0x1.5d28da2c4612dp-56, -0x1.760b1e2e3f81ep-55},
+    {0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1, -0x1.26d19b9ff8d82p-57, -0x1.562172a361fd3p-56},
+    {0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1, -0x1.e2718d26ed688p-60, 0x1.87df6378811c7p-55},
+    {-0x1.8338bfdf4415ap-170, -0x1.0000000000000p+0, 0x1.d89cd9128a504p-225, 0x0.0p+0},
+    {-0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1, 0x1.e2718d26ed688p-60, 0x1.87df6378811c7p-55},
+    {-0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1, 0x1.26d19b9ff8d82p-57, -0x1.562172a361fd3p-56},
+    {-0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1, 0x1.5d28da2c4612dp-56, -0x1.760b1e2e3f81ep-55},
+    {-0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1, 0x1.72cedd3d5a610p-57, -0x1.457e610231ac2p-56},
+    {-0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1, -0x1.e0d891d3c6841p-58, 0x1.6e0b1757c8d07p-56},
+    {-0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1, -0x1.b25dd267f6600p-55, -0x1.9f630e8b6dac8p-60},
+    {-0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1, -0x1.8076a2cfdc6b3p-57, 0x1.2c5e12ed1336dp-55},
+    {-0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1, 0x1.bdd3413b26456p-55, 0x1.bdd3413b26456p-55},
+    {-0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1, 0x1.2c5e12ed1336dp-55, -0x1.8076a2cfdc6b3p-57},
+    {-0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1, -0x1.9f630e8b6dac8p-60, -0x1.b25dd267f6600p-55},
+    {-0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2, 0x1.6e0b1757c8d07p-56, -0x1.e0d891d3c6841p-58},
+    {-0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2, -0x1.457e610231ac2p-56, 0x1.72cedd3d5a610p-57},
+    {-0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2, -0x1.760b1e2e3f81ep-55, 0x1.5d28da2c4612dp-56},
+    {-0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3, -0x1.562172a361fd3p-56, 0x1.26d19b9ff8d82p-57},
+    {-0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4, 0x1.87df6378811c7p-55, 0x1.e2718d26ed688p-60},
+    {-0x1.0000000000000p+0, 0x1.913547f3b9882p-168, 0x0.0p+0, -0x1.589d68b379ef1p-222},
+    {-0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4, 0x1.87df6378811c7p-55, -0x1.e2718d26ed688p-60},
+    {-0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3, -0x1.562172a361fd3p-56, -0x1.26d19b9ff8d82p-57},
+    {-0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2, -0x1.760b1e2e3f81ep-55, -0x1.5d28da2c4612dp-56},
+    {-0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2, -0x1.457e610231ac2p-56, -0x1.72cedd3d5a610p-57},
+    {-0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2, 0x1.6e0b1757c8d07p-56, 0x1.e0d891d3c6841p-58},
+    {-0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1, -0x1.9f630e8b6dac8p-60, 0x1.b25dd267f6600p-55},
+    {-0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1, 0x1.2c5e12ed1336dp-55, 0x1.8076a2cfdc6b3p-57},
+    {-0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1, 0x1.bdd3413b26456p-55, -0x1.bdd3413b26456p-55},
+    {-0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1, -0x1.8076a2cfdc6b3p-57, -0x1.2c5e12ed1336dp-55},
+    {-0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1, -0x1.b25dd267f6600p-55, 0x1.9f630e8b6dac8p-60},
+    {-0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1, -0x1.e0d891d3c6841p-58, -0x1.6e0b1757c8d07p-56},
+    {-0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1, 0x1.72cedd3d5a610p-57, 0x1.457e610231ac2p-56},
+    {-0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1, 0x1.5d28da2c4612dp-56, 0x1.760b1e2e3f81ep-55},
+    {-0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1, 0x1.26d19b9ff8d82p-57, 0x1.562172a361fd3p-56},
+    {-0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1, 0x1.e2718d26ed688p-60, -0x1.87df6378811c7p-55},
+};
 static __constant__ double EXP_P[12] = {0x1.0000000000000p+0, 0x1.0000000000000p+0, 0x1.0000000000012p-1,
                                         0x1.555555555555bp-3, 0x1.555555554e892p-5, 0x1.111111110ef9cp-7,
                                         0x1.6c16c189ae5d5p-10, 0x1.a01a01b24ec34p-13, 0x1.a0198d2d43afbp-16,
@@ -322,21 +392,21 @@ template <> struct Math<double> {
     __device__ static __forceinline__ double track_min(double a, double b) { return ::fmin(a, b); }
     __device__ static __forceinline__ double neg2log(double u) { return -2.0 * log(u); }
     __device__ static __forceinline__ double half_log(double x) { return 0.5 * log(x); }
-    // sin/cos of 2 pi u for u in [0, 1]: 4u = n + r exactly with |r| <= 1/2,
-    // so 2 pi u = n pi/2 + r pi/2; the quadrant n mod 4 swaps and negates the
-    // two polynomials.
+    // sin/cos of 2 pi u for u in [0, 1]: 64u = j + r exactly (|r| <= 1/2), and
+    // with A = 2 pi j/64 from the table and B = 2 pi r/64,
+    // sin(A + B) = sin A + (sin A (cos B - 1) + cos A sin B) and
+    // cos(A + B) = cos A + (cos A (cos B - 1) - sin A sin B), the table values
+    // carried with their remainders.
     __device__ static __forceinline__ void sincos2pi(double u, double &s, double &c) {
-        const double x = 4.0 * u;
+        const double x = 64.0 * u;
         const double n = __builtin_rint(x);
         const double r = x - n;
         const double w = r * r;
-        const double sr = r * horner(SIN_S, w);
-        const double cr = horner(COS_C, w);
-        const int q = (int)n;
-        const double a = (q & 1) ? cr : sr;
-        const double b = (q & 1) ? sr : cr;
-        s = (q & 2) ? -a : a;
-        c = ((q + 1) & 2) ? -b : b;
+        const double sb = r * horner(SIN64_S, w);
+        const double cm1 = w * horner(COS64_M, w);
+        const SinCosEntry e = SINCOS_TAB[(int)n & 63];
+        s = e.s + (e.s_lo + __builtin_fma(e.s, cm1, e.c * sb));
+        c = e.c + (e.c_lo + __builtin_fma(e.c, cm1, -(e.s * sb)));
     }
     // All 32 bits of a Philox word, centred: (w + 0.5) * 2^-32, exact in FP64
     // (0.5 is an inline operand and 2^-32 a 32-bit literal; 2^-33 as the

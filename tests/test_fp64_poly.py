@@ -5,10 +5,11 @@ The coefficients and the log table are read from the header as written, so a
 mistyped digit, a dropped term or a reversed order fails here.  The bounds are
 the near-minimax fit error plus the rounding of the coefficients to double:
   log1p: r + r^2 Q(r) vs log1p(r) on |r| <= 0.009                   < 2e-17 relative
-  sin:   S(w) vs sin(pi/2 sqrt w)/sqrt w on w in [0, 1/4]           < 2.5e-16 absolute
-  cos:   C(w) vs cos(pi/2 sqrt w)                                   < 6e-17 absolute
+  sin:   S(w) vs sin(pi/32 sqrt w)/sqrt w on w in [0, 1/4]          < 6e-17 relative
+  cos:   w M(w) vs cos(pi/32 sqrt w) - 1                            < 2e-19 absolute
   exp:   E(r) vs e^r on r in [-0.35, 0.35]                          < 1e-17 relative
-the log table (each entry's interval of m gives |m invc - 1| <= 0.009, invc
+the sin/cos table (hi + lo = sin, cos of 2 pi j/64 to 2^-100), the log
+table (each entry's interval of m gives |m invc - 1| <= 0.009, invc
 has at most 7 significant bits so that m invc - 1 is exact in one fma, and
 lhi + llo = -log(invc) to 2^-76), and the split of ln 2 into LN2_HI (32
 significant bits) + LN2_LO.
@@ -87,14 +88,31 @@ def test_log_table():
 
 def test_sin_cos_polynomials():
     mp.mp.dps = 50
-    cs, cc = header_array("SIN_S"), header_array("COS_C")
+    cs, cm = header_array("SIN64_S"), header_array("COS64_M")
     ws, wc = 0, 0
     for w in grid(mp.mpf(1) / 4)[1:]:
         r = mp.sqrt(w)
-        ws = max(ws, abs(poly(cs, w) - mp.sin(mp.pi / 2 * r) / r))
-        wc = max(wc, abs(poly(cc, w) - mp.cos(mp.pi / 2 * r)))
-    assert ws < 2.5e-16, ws
-    assert wc < 6e-17, wc
+        ws = max(ws, abs(poly(cs, w) * r / mp.sin(mp.pi / 32 * r) - 1))
+        wc = max(wc, abs(w * poly(cm, w) - (mp.cos(mp.pi / 32 * r) - 1)))
+    assert ws < 6e-17, ws        # the rounding of pi/32 to double (<= 2^-54) dominates
+    assert wc < 2e-19, wc
+
+
+def header_sincos_table():
+    text = open(HEADER).read()
+    m = re.search(r"\bSINCOS_TAB\[(\d+)\]\s*=\s*\{(.*?)\n\};", text, re.S)
+    assert m
+    rows = re.findall(r"\{([^}]*)\}", m.group(2))
+    assert len(rows) == int(m.group(1)) == 64
+    return [tuple(float.fromhex(x.strip()) for x in row.split(",")) for row in rows]
+
+
+def test_sincos_table():
+    mp.mp.dps = 50
+    for j, (sh, ch, sl, cl) in enumerate(header_sincos_table()):
+        a = 2 * mp.pi * j / 64
+        assert abs(mp.mpf(sh) + mp.mpf(sl) - mp.sin(a)) < mp.mpf(2) ** -100, j
+        assert abs(mp.mpf(ch) + mp.mpf(cl) - mp.cos(a)) < mp.mpf(2) ** -100, j
 
 
 def test_exp_polynomial():
@@ -121,7 +139,8 @@ def test_fit_script_reproduces_header():
     spec = importlib.util.spec_from_file_location("fit_fp64_poly", FIT)
     fit = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(fit)
-    for name in ("SIN_S", "COS_C", "EXP_P", "LOG1P_Q"):
+    for name in ("SIN64_S", "COS64_M", "EXP_P", "LOG1P_Q"):
         assert fit.coefficients(name) == header_array(name), name
+    assert fit.sincos_table() == header_sincos_table()
     tab, _ = fit.log_table()
     assert [(a, float(b), c) for a, b, c in tab] == header_log_table()
