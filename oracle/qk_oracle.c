@@ -257,9 +257,21 @@ static void simulate_one(const scheme_t *s, double S0, double v0, uint64_t path,
                          double *S_store, int64_t ld, double *S_T, double *v_T,
                          double *S_sum, double *S_max, double *S_min) {
     double S = S0, v = v0, sum = 0.0, mx = S0, mn = S0;
+    /* Heston Euler / Milstein use two words per step: steps 2i and 2i+1 take
+     * words (0, 1) and (2, 3) of block (path, i, 0); QE and Bates use a block
+     * per step (word 2: QE exponential branch, word 3: Poisson uniform). */
+    const int paired = s->model == QKO_HESTON && (s->scheme == QKO_EULER || s->scheme == QKO_MILSTEIN);
     for (int32_t t = 0; t < s->n_steps; ++t) {
         uint32_t w[4];
-        block_words(s->seed, path, (uint32_t)t, 0u, w);
+        if (paired) {
+            uint32_t b[4];
+            block_words(s->seed, path, (uint32_t)t >> 1, 0u, b);
+            w[0] = b[(t & 1) * 2];
+            w[1] = b[(t & 1) * 2 + 1];
+            w[2] = w[3] = 0u;
+        } else {
+            block_words(s->seed, path, (uint32_t)t, 0u, w);
+        }
         heston_step(s, w, &S, &v);
         if (s->model == QKO_BATES) bates_jump(s, w, path, (uint32_t)t, &S);
         if (S_store) S_store[(int64_t)t * ld] = S;

@@ -63,18 +63,19 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
         sm[l] = 0.0;
     }
 
-    for (int t = 0; t < n_steps; ++t) {
-        const Words w = philox(p0, p1, (uint32_t)t, 0u, key0, key1);
-        const Real u0 = M::uniform(w.w0), u1 = M::uniform(w.w1);
+    // One step from the words (a0, a1) for the Box-Muller pair, a2 for the QE
+    // exponential branch and a3 for the first Bates Poisson uniform.
+    auto step = [&](int t, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
+        const Real u0 = M::uniform(a0), u1 = M::uniform(a1);
         Real sn, cs;
         M::sincos2pi(u1, sn, cs);
         const Real R = M::sqrt(M::neg2log(u0));
         const Real z1 = R * cs, z2 = R * sn;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            Real inc = heston_step<Real, SCHEME>(cl[l], z1, z2, w.w2, v[l]);
+            Real inc = heston_step<Real, SCHEME>(cl[l], z1, z2, a2, v[l]);
             if constexpr (MODEL == MODEL_BATES) {
-                const Real u3 = M::uniform(w.w3);
+                const Real u3 = M::uniform(a3);
                 if (u3 >= cl[l].eml_dt) inc += bates_jump_slow<Real>(cl[l], u3, p0, p1, (uint32_t)t, key0, key1);
             }
             if constexpr (PATHDEP) {
@@ -88,6 +89,25 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
             if constexpr (STORE) {
                 if (active) store[(int64_t)t * n_paths + path] = S[l];
             }
+        }
+    };
+    if constexpr (MODEL == MODEL_HESTON && (SCHEME == SCHEME_EULER || SCHEME == SCHEME_MILSTEIN)) {
+        // Two words per step: steps 2i and 2i+1 take words (0, 1) and (2, 3)
+        // of block (path, i, slot 0).
+        int t = 0;
+        for (; t + 1 < n_steps; t += 2) {
+            const Words w = philox(p0, p1, (uint32_t)t >> 1, 0u, key0, key1);
+            step(t, w.w0, w.w1, 0u, 0u);
+            step(t + 1, w.w2, w.w3, 0u, 0u);
+        }
+        if (t < n_steps) {
+            const Words w = philox(p0, p1, (uint32_t)t >> 1, 0u, key0, key1);
+            step(t, w.w0, w.w1, 0u, 0u);
+        }
+    } else {
+        for (int t = 0; t < n_steps; ++t) {
+            const Words w = philox(p0, p1, (uint32_t)t, 0u, key0, key1);
+            step(t, w.w0, w.w1, w.w2, w.w3);
         }
     }
 

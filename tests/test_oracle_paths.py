@@ -202,7 +202,17 @@ def test_greek_put_call_parity(oracle):
 
     assert abs((gc.delta - gp.delta) - 1.0) <= tol(1.0, 3.3)
     assert abs(gc.gamma - gp.gamma) <= tol(1.0, 3.3)
-    assert abs(gc.vega - gp.vega) <= tol(1.0, 3.3)
+    # Per path C - P = e^{-rT}(S_T - K), so vega(C) - vega(P) is the bumped
+    # difference of mean S_T, zero in expectation; its noise is that of the
+    # per-path difference of S_T between the two variance bumps (common
+    # random numbers), divided by the bump width -- far above the price SE.
+    vol0 = math.sqrt(m.v0)
+    up, dn = M(oracle, scheme=1, n_steps=100, v0=(vol0 + 0.02) ** 2, r=0.05, q=0.0, kappa=2.0, theta=0.04,
+               sigma=0.3, rho=-0.7), M(oracle, scheme=1, n_steps=100, v0=(vol0 - 0.02) ** 2, r=0.05, q=0.0,
+                                       kappa=2.0, theta=0.04, sigma=0.3, rho=-0.7)
+    d = oracle.simulate(up, n)["S_T"] - oracle.simulate(dn, n)["S_T"]
+    se_vega_diff = 100.0 * math.exp(-r * T) * d.std() / math.sqrt(n) / (2 * 0.02)
+    assert abs(gc.vega - gp.vega) <= 4.0 * se_vega_diff + 1e-9, (gc.vega - gp.vega, se_vega_diff)
     # vomma parity is noise-dominated at this path count; the GPU suite checks it
     # at the reference's own size (1M paths, 365 steps)
     assert abs(gc.vanna - gp.vanna) <= tol(1.0, 3.3)
