@@ -275,7 +275,7 @@ def serving(engine, n_requests=4096, fp64_fraction=0.25):
         if done % 128 == 0:
             progress("serving: plain FP64 %d/%d" % (done, len(idx)))
         if s.se > 0:
-            (err_am if p.type == qk.AMERICAN else err).append(abs(burst[i].stats.price - s.price) / s.se)
+            (err_am if p.type == qk.AMERICAN else err).append((abs(burst[i].stats.price - s.price) / s.se, i))
     wall = time.perf_counter() - t0
     out["fp64_plain"] = {"requests": len(idx), "wall_s": wall, "path_steps_per_s": ps64 / wall}
     out["fp32_batched_over_fp64_plain"] = out["burst"]["path_steps_per_s"] / out["fp64_plain"]["path_steps_per_s"]
@@ -283,8 +283,8 @@ def serving(engine, n_requests=4096, fp64_fraction=0.25):
         e = sorted(e)
         if not e:
             return None
-        return {"n": len(e), "max": e[-1], "p99": e[int(0.99 * (len(e) - 1))],
-                "within_1se": sum(x <= 1.0 for x in e) / len(e)}
+        return {"n": len(e), "max": e[-1][0], "max_request": e[-1][1], "p99": e[int(0.99 * (len(e) - 1))][0],
+                "within_1se": sum(x <= 1.0 for x, _ in e) / len(e)}
     # FP32 batched vs FP64 plain, same random numbers.  For American requests
     # the two precisions may settle on different exercise policies, so their
     # difference also holds the policy's own sampling noise, which the
