@@ -5,7 +5,8 @@
 //               Replaces MarkovianIntegrationKernel (MSolver.hpp:98-130) and the
 //               two reductions of OptionPricer::accumulate_batch_payoffs
 //               (OptionPricer.hpp:129-146).  L lanes (bumped contracts) per
-//               thread share one Philox block per step (common random numbers).
+//               thread share each step's Philox words (common random numbers);
+//               Heston Euler/Milstein take two steps from one block.
 //  k_merge      fixed-order Chan merge of tile moments + FP64 finalisation
 //               (OptionPricer.hpp:152-172, :204-249).
 //  k_greeks     FP64 central differences of lane prices (MCEngine.hpp:586-730).
