@@ -54,6 +54,22 @@ def test_paths_fp64_match_oracle(engine, oracle, model, scheme):
     assert np.median(rel) < 1e-14, np.median(rel)
 
 
+@pytest.mark.parametrize("path0", [37, 2 ** 32 - 100])
+def test_paths_at_counter_offsets_match_oracle(engine, oracle, path0):
+    """qk_simulate_paths over [path0, path0 + 200): the random stream is keyed by
+    the global path index, so an offset window equals the oracle's same window
+    (and the matching slice of a window from 0).  path0 = 2^32 - 100 puts the
+    carry into the counter's high word inside a 64-path tile."""
+    for model, scheme in ((qk.HESTON, qk.QE), (qk.HESTON, qk.EULER), (qk.BATES, qk.MILSTEIN)):
+        c = base_contract(model, scheme, qk.FP64, n_paths=200, n_steps=21)
+        g = engine.simulate_paths(c, path0=path0)
+        o = oracle.simulate(omodel(oracle, c), c.n_paths, path0=path0, store=True)["paths"]
+        assert np.abs(g - o).max() / np.abs(o).max() < 1e-12, (model, scheme, path0)
+    c = base_contract(qk.HESTON, qk.QE, qk.FP64, n_paths=100, n_steps=21)
+    whole = engine.simulate_paths(base_contract(qk.HESTON, qk.QE, qk.FP64, n_paths=137, n_steps=21))
+    assert np.array_equal(engine.simulate_paths(c, path0=37), whole[:, 37:137])
+
+
 @pytest.mark.parametrize("model,scheme", MODELS)
 def test_paths_fp32_match_oracle(engine, oracle, model, scheme):
     c = base_contract(model, scheme, qk.FP32)
