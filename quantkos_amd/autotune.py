@@ -12,17 +12,21 @@ Knobs:
     per step; fewer keep more waves resident and waste no padded lanes.
   * tile schedule of the request server: path-steps per work item (the tiles a
     wavefront pulls at a time), resident blocks per CU of a batch launch, and
-    batch launches in flight.
+    batch launches in flight.  Chosen by burst throughput first, then by p50
+    and p99 latency under Poisson arrivals among the candidates within 3 % of
+    the best throughput (larger items raise throughput little but make
+    batches coarser, which lengthens the typical request's wait).
 
 The reference has no counterpart: its launch shape is whatever Kokkos'
 RangePolicy picks (MSolver.hpp:195-232).
 
-    python -m quantkos_amd.autotune [--quick] [--out FILE]
+    python -m quantkos_amd.autotune [--quick] [--server-only] [--out FILE]
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import time
 
@@ -39,6 +43,8 @@ LANES_PER_THREAD = (1, 2, 4)
 LANE_COUNTS = (2, 4, 8, 12, 13, 15, 16)
 SERVER_CANDIDATES = [dict(item_path_steps=ips, blocks_per_cu=bpc, max_inflight=mi)
                      for ips in (65536, 131072, 524288) for bpc in (4, 8) for mi in (2, 3, 4)]
+# Burst-throughput band within which the server schedule is chosen by latency.
+SERVER_RATE_TOLERANCE = 0.03
 
 
 def load(path: str = DEFAULT_PATH):
@@ -125,9 +131,26 @@ def tune_lanes(engine, precisions=(_lib.FP32, _lib.FP64), schemes=(_lib.EULER, _
     return entries
 
 
-def tune_server(engine, n_requests=384, candidates=None, log=print):
-    """Burst throughput (path-steps/s) of the first `n_requests` requests of the
-    config-4 stream (FP32) under each server schedule candidate."""
+def pick_server(rates: dict, latencies: dict, tolerance: float = SERVER_RATE_TOLERANCE):
+    """Server schedule choice: of the candidates whose burst throughput is
+    within `tolerance` of the best, the one with the smallest geometric mean
+    of p50 and p99 latency under Poisson arrivals (`latencies` maps a
+    candidate to (p50, p99)); ties go to the first listed.  Candidates
+    without a latency entry count only when none has one."""
+    top = max(rates.values())
+    short = [k for k, v in rates.items() if v >= (1.0 - tolerance) * top]
+    timed = [k for k in short if k in latencies]
+    if not timed:
+        return pick_best({k: -rates[k] for k in short})
+    return pick_best({k: math.sqrt(latencies[k][0] * latencies[k][1]) for k in timed})
+
+
+def tune_server(engine, n_requests=384, candidates=None, latency_requests=1024, log=print):
+    """Server schedule: burst throughput (path-steps/s) of the first
+    `n_requests` requests of the config-4 stream (FP32) under each candidate,
+    then p50/p99 latency of a `latency_requests` stream with Poisson arrivals
+    at 0.8x the best burst capacity under each candidate within
+    SERVER_RATE_TOLERANCE of that capacity (pick_server chooses)."""
     from .engine import Server
     from . import workloads
     cs, ps, _ = workloads.config4_stream(n=n_requests, seed=2024, precision=_lib.FP32)
@@ -145,22 +168,55 @@ def tune_server(engine, n_requests=384, candidates=None, log=print):
             key = tuple(sorted(cand.items()))
             rates[key] = total / wall
             log("server %s -> %.3e path-steps/s" % (cand, total / wall))
+        latencies = {}
+        if latency_requests:
+            top = max(rates.values())
+            cl, pl, _ = workloads.config4_stream(n=latency_requests, seed=2025, precision=_lib.FP32)
+            rate = 0.8 * top / (workloads.path_steps(cl) / len(cl))
+            cl, pl, arr = workloads.config4_stream(n=latency_requests, seed=2025, precision=_lib.FP32,
+                                                    rate=rate)
+            for key, v in rates.items():
+                if v < (1.0 - SERVER_RATE_TOLERANCE) * top:
+                    continue
+                with Server(engine, **dict(key)) as srv:
+                    lat = sorted(r.latency_s for r in srv.replay(cl, pl, arr))
+                q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))] * 1e3
+                latencies[key] = (q(0.50), q(0.99))
+                log("server %s at %.0f req/s -> p50 %.1f ms, p99 %.1f ms"
+                    % (dict(key), rate, latencies[key][0], latencies[key][1]))
     finally:
         engine.tuning = saved
-    best = pick_best({k: -v for k, v in rates.items()})
-    return dict(best), {json.dumps(dict(k)): v for k, v in rates.items()}
+    best = pick_server(rates, latencies)
+    return (dict(best), {json.dumps(dict(k)): v for k, v in rates.items()},
+            {json.dumps(dict(k)): [round(x, 3) for x in v] for k, v in latencies.items()})
+
+
+def tune_server_only(engine, path: str = DEFAULT_PATH, log=print) -> dict:
+    """Re-tunes the server schedule of the table at `path`, keeping its lane
+    entries."""
+    table = load(path)
+    if table is None:
+        raise FileNotFoundError(path)
+    server, rates, lat = tune_server(engine, log=log)
+    table.update(server=server, server_path_steps_per_s=rates, server_latency_ms=lat,
+                 created=time.strftime("%Y-%m-%d %H:%M:%S"))
+    validate(table)
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1)
+    return table
 
 
 def autotune(engine, out: str = DEFAULT_PATH, quick: bool = False, log=print) -> dict:
     if quick:
         lanes = tune_lanes(engine, precisions=(_lib.FP32,), schemes=(_lib.MILSTEIN,), lane_counts=(4, 13),
                            n_paths=1 << 16, n_steps=32, repeats=1, log=log)
-        server, rates = tune_server(engine, n_requests=64, candidates=SERVER_CANDIDATES[:2], log=log)
+        server, rates, lat = tune_server(engine, n_requests=64, candidates=SERVER_CANDIDATES[:2],
+                                         latency_requests=64, log=log)
     else:
         lanes = tune_lanes(engine, log=log)
-        server, rates = tune_server(engine, log=log)
+        server, rates, lat = tune_server(engine, log=log)
     table = {"arch": ARCH, "lanes": lanes, "server": server, "server_path_steps_per_s": rates,
-             "created": time.strftime("%Y-%m-%d %H:%M:%S")}
+             "server_latency_ms": lat, "created": time.strftime("%Y-%m-%d %H:%M:%S")}
     validate(table)
     if out:
         os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
@@ -173,11 +229,17 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=DEFAULT_PATH)
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--server-only", action="store_true",
+                    help="re-tune the server schedule of the table at --out, keeping its lane entries")
     ap.add_argument("--device", type=int, default=0)
     args = ap.parse_args()
     from .engine import Engine
     with Engine(args.device, tuning=None) as eng:
-        autotune(eng, args.out, args.quick, log=lambda m: print(m, flush=True))
+        log = lambda m: print(m, flush=True)
+        if args.server_only:
+            tune_server_only(eng, args.out, log=log)
+        else:
+            autotune(eng, args.out, args.quick, log=log)
     print("wrote", args.out)
 
 
