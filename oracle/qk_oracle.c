@@ -49,8 +49,14 @@ void qko_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+/* FP32 map (u24 != 0): w rounded to float (nearest even), times 2^-32, plus
+ * 2^-33, rounded to float, in (0, 1]; the product is exact, so this is the
+ * device's fmaf.  FP64 map: (w + 1/2) 2^-32, exact, in (0, 1). */
 double qko_uniform(uint32_t w, int u24) {
-    if (u24) return ((double)(w >> 8) + 0.5) * 0x1.0p-24;
+    if (u24) {
+        const float f = (float)w;
+        return (double)(float)(f * 0x1.0p-32f + 0x1.0p-33f);
+    }
     return ((double)w + 0.5) * 0x1.0p-32;
 }
 
@@ -257,10 +263,15 @@ static void simulate_one(const scheme_t *s, double S0, double v0, uint64_t path,
                          double *S_store, int64_t ld, double *S_T, double *v_T,
                          double *S_sum, double *S_max, double *S_min) {
     double S = S0, v = v0, sum = 0.0, mx = S0, mn = S0;
-    /* Heston Euler / Milstein use two words per step: steps 2i and 2i+1 take
-     * words (0, 1) and (2, 3) of block (path, i, 0); QE and Bates use a block
-     * per step (word 2: QE exponential branch, word 3: Poisson uniform). */
-    const int paired = s->model == QKO_HESTON && (s->scheme == QKO_EULER || s->scheme == QKO_MILSTEIN);
+    /* Words of step t.  Heston Euler / Milstein: two per step, steps 2i and
+     * 2i+1 take words (0, 1) and (2, 3) of block (path, i, 0).  Heston QE:
+     * three per step (the third for the exponential branch), words 3t..3t+2
+     * of the sequence that lists blocks (path, 0, 0), (path, 1, 0), ... word
+     * by word.  Bates: block (path, t, 0), word 2 for QE, word 3 the first
+     * Poisson uniform. */
+    const int heston = s->model == QKO_HESTON;
+    const int paired = heston && (s->scheme == QKO_EULER || s->scheme == QKO_MILSTEIN);
+    const int triple = heston && s->scheme == QKO_QE;
     for (int32_t t = 0; t < s->n_steps; ++t) {
         uint32_t w[4];
         if (paired) {
@@ -269,6 +280,13 @@ static void simulate_one(const scheme_t *s, double S0, double v0, uint64_t path,
             w[0] = b[(t & 1) * 2];
             w[1] = b[(t & 1) * 2 + 1];
             w[2] = w[3] = 0u;
+        } else if (triple) {
+            uint32_t b[8];
+            const uint32_t k = 3u * (uint32_t)t;
+            block_words(s->seed, path, k / 4u, 0u, b);
+            block_words(s->seed, path, k / 4u + 1u, 0u, b + 4);
+            for (int j = 0; j < 3; ++j) w[j] = b[k % 4u + (uint32_t)j];
+            w[3] = 0u;
         } else {
             block_words(s->seed, path, (uint32_t)t, 0u, w);
         }

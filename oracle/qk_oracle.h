@@ -16,14 +16,20 @@
  * independently, so per-path results can be compared:
  *
  *   key      = (lo32(seed), hi32(seed))
- *   block    = Philox4x32-10(counter = (lo32(path), hi32(path), step, slot), key)
- *   U(w)     = (w + 0.5) * 2^-32                 (FP64 runs)
- *            = ((w >> 8) + 0.5) * 2^-24          (FP32 runs, "u24")
- *   step t, slot 0 -> words w0..w3:
- *     Z1 = sqrt(-2 ln U(w0)) cos(2 pi U(w1)),  Z2 = sqrt(-2 ln U(w0)) sin(2 pi U(w1))
+ *   block(path, b, slot) = Philox4x32-10(counter = (lo32(path), hi32(path), b, slot), key)
+ *   U(w)     = (w + 0.5) * 2^-32                            (FP64 runs)
+ *            = fl32(fl32(w) * 2^-32 + 2^-33), in (0, 1]     (FP32 runs, "u24":
+ *              w rounded to a 24-bit significand, nearest even)
+ *   words (a0, a1, a2, a3) of step t:
+ *     Heston Euler / Milstein: two per step, steps 2i, 2i+1 take words (0, 1)
+ *                        and (2, 3) of block(path, i, 0)
+ *     Heston QE:         three per step, words 3t, 3t+1, 3t+2 of the sequence
+ *                        block(path, 0, 0) w0..w3, block(path, 1, 0) w0..w3, ...
+ *     Bates:             the four words of block(path, t, 0)
+ *   Z1 = sqrt(-2 ln U(a0)) cos(2 pi U(a1)),  Z2 = sqrt(-2 ln U(a0)) sin(2 pi U(a1))
  *     Euler / Milstein:  Z_v = Z1,  Z_S = rho Z1 + sqrt(1-rho^2) Z2
- *     QE:                Z_V = Z1,  Z_indep = Z2,  exponential-branch uniform = U(w2)
- *     Bates:             first Poisson uniform = U(w3); further uniforms e_j are
+ *     QE:                Z_V = Z1,  Z_indep = Z2,  exponential-branch uniform = U(a2)
+ *     Bates:             first Poisson uniform = U(a3); further uniforms e_j are
  *                        word (j mod 4) of block(path, t, 1 + j/4); the jump
  *                        normal after N jumps is sqrt(-2 ln e_N) cos(2 pi e_{N+1}).
  */
@@ -49,7 +55,7 @@ enum { QKO_CALL = 0, QKO_PUT = 1 };
 typedef struct {
     int32_t model;           /* QKO_HESTON / QKO_BATES */
     int32_t scheme;          /* QKO_EULER / QKO_MILSTEIN / QKO_QE */
-    int32_t fp32_semantics;  /* 1: 24-bit uniforms and the FP32 eps constants */
+    int32_t fp32_semantics;  /* 1: FP32 uniforms (u24) and the FP32 eps constants */
     int32_t n_steps;
     uint64_t seed;
     double S0, v0, r, q, T;

@@ -83,9 +83,12 @@ template <> struct Math<float> {
         s = __builtin_amdgcn_sinf(u);
         c = __builtin_amdgcn_cosf(u);
     }
-    // Top 24 bits of a Philox word, centred: ((w >> 8) + 0.5) * 2^-24, exact in FP32.
+    // A Philox word rounded to FP32 (round to nearest even), scaled by 2^-32
+    // and shifted by 2^-33: RN(RN(w) 2^-32 + 2^-33), in (0, 1].  Two
+    // instructions (convert, fma); the spacing is that of FP32 itself, 2^-24
+    // near 1 and finer towards 0 (smallest value 2^-33).
     __device__ static __forceinline__ float uniform(uint32_t w) {
-        return __builtin_fmaf((float)(w >> 8), 0x1.0p-24f, 0x1.0p-25f);
+        return __builtin_fmaf((float)w, 0x1.0p-32f, 0x1.0p-33f);
     }
 };
 

@@ -6,7 +6,8 @@
 //               two reductions of OptionPricer::accumulate_batch_payoffs
 //               (OptionPricer.hpp:129-146).  L lanes (bumped contracts) per
 //               thread share each step's Philox words (common random numbers);
-//               Heston Euler/Milstein take two steps from one block.
+//               Heston Euler/Milstein take two steps from one block, Heston QE
+//               four steps from three.
 //  k_merge      fixed-order Chan merge of tile moments + FP64 finalisation
 //               (OptionPricer.hpp:152-172, :204-249).
 //  k_greeks     FP64 central differences of lane prices (MCEngine.hpp:586-730).
@@ -104,6 +105,61 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
         if (t < n_steps) {
             const Words w = philox(p0, p1, (uint32_t)t >> 1, 0u, key0, key1);
             step(t, w.w0, w.w1, 0u, 0u);
+        }
+    } else if constexpr (MODEL == MODEL_HESTON && SCHEME == SCHEME_QE) {
+        // Three words per step, taken in order from the blocks (path, b, 0),
+        // b = 0, 1, ...: step t uses words 3t, 3t+1, 3t+2 of that sequence
+        // (word k = word k mod 4 of block k / 4), so steps 4i..4i+3 share
+        // blocks 3i..3i+2.
+        if constexpr (sizeof(Real) == 8) {
+            // FP64: one step per iteration (four steps unrolled exceed the
+            // register budget), carrying the block that holds word 3t.
+            Words cur{0u, 0u, 0u, 0u};
+            for (int t = 0; t < n_steps; ++t) {
+                const uint32_t k = 3u * (uint32_t)t, r = k & 3u;
+                uint32_t a0, a1, a2;
+                if (r == 0u) {
+                    cur = philox(p0, p1, k >> 2, 0u, key0, key1);
+                    a0 = cur.w0; a1 = cur.w1; a2 = cur.w2;
+                } else if (r == 1u) {
+                    a0 = cur.w1; a1 = cur.w2; a2 = cur.w3;
+                } else {
+                    const Words nx = philox(p0, p1, (k >> 2) + 1u, 0u, key0, key1);
+                    if (r == 3u) {
+                        a0 = cur.w3; a1 = nx.w0; a2 = nx.w1;
+                    } else {
+                        a0 = cur.w2; a1 = cur.w3; a2 = nx.w0;
+                    }
+                    cur = nx;
+                }
+                step(t, a0, a1, a2, 0u);
+            }
+        } else {
+            int t = 0;
+            uint32_t b = 0;
+            for (; t + 3 < n_steps; t += 4, b += 3) {
+                const Words x = philox(p0, p1, b, 0u, key0, key1);
+                step(t, x.w0, x.w1, x.w2, 0u);
+                const Words y = philox(p0, p1, b + 1, 0u, key0, key1);
+                step(t + 1, x.w3, y.w0, y.w1, 0u);
+                const Words z = philox(p0, p1, b + 2, 0u, key0, key1);
+                step(t + 2, y.w2, y.w3, z.w0, 0u);
+                step(t + 3, z.w1, z.w2, z.w3, 0u);
+            }
+            for (; t < n_steps; ++t) {      // at most 3 steps; word 3t is word 0, 3 or 2 of its block
+                const uint32_t k = 3u * (uint32_t)t;
+                const Words x = philox(p0, p1, k >> 2, 0u, key0, key1);
+                uint32_t a0 = x.w0, a1 = x.w1, a2 = x.w2;
+                if ((k & 3u) != 0u) {
+                    const Words y = philox(p0, p1, (k >> 2) + 1u, 0u, key0, key1);
+                    if ((k & 3u) == 3u) {
+                        a0 = x.w3; a1 = y.w0; a2 = y.w1;
+                    } else {
+                        a0 = x.w2; a1 = x.w3; a2 = y.w0;
+                    }
+                }
+                step(t, a0, a1, a2, 0u);
+            }
         }
     } else {
         for (int t = 0; t < n_steps; ++t) {

@@ -124,7 +124,7 @@ def test_prices_fp32_within_se_of_oracle(engine, oracle, model, scheme):
 
 def test_fp32_matches_fp64_gpu_on_same_numbers(engine):
     """Config-1 shape at reduced size: FP32 and FP64 GPU runs share the random
-    stream (24-bit vs 32-bit uniforms of the same Philox words)."""
+    stream (FP32 and FP64 uniform maps of the same Philox words)."""
     c32 = base_contract(qk.HESTON, qk.QE, qk.FP32, n_paths=200_000, n_steps=365, S0=1.0, v0=0.04,
                         r=0.03, q=0.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7)
     c64 = base_contract(qk.HESTON, qk.QE, qk.FP64, n_paths=200_000, n_steps=365, S0=1.0, v0=0.04,
@@ -185,7 +185,7 @@ def test_american_matches_oracle(engine, oracle, precision, model, scheme):
     """Same paths, same regression.  The Laguerre Gram at early dates is badly
     conditioned (X = S/K clusters), so summation-order rounding moves individual
     coefficients; the fitted continuation values and the price must agree:
-    FP64 within 0.02 SE, FP32 (24-bit uniforms, FP32 paths) within 0.5 SE."""
+    FP64 within 0.02 SE, FP32 (FP32 uniforms, FP32 paths) within 0.5 SE."""
     c = base_contract(model, scheme, precision, n_paths=20_000, n_steps=50, price_scale=100.0)
     g, gco = engine.price_american(c, qk.PUT, 1.1, want_coeffs=True)
     o, oco = oracle.price_american(omodel(oracle, c), c.n_paths, qk.PUT, 1.1, 100.0, want_coeffs=True)

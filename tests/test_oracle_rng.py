@@ -20,13 +20,33 @@ def test_philox_known_answers(oracle):
         assert tuple(oracle.philox(ctr, key)) == want
 
 
-def test_uniform_maps_open_interval(oracle):
-    assert oracle.lib().qko_uniform(0, 0) == 2.0 ** -33
-    assert oracle.lib().qko_uniform(0xFFFFFFFF, 0) == 1.0 - 2.0 ** -33
-    assert oracle.lib().qko_uniform(0, 1) == 2.0 ** -25
-    assert oracle.lib().qko_uniform(0xFFFFFFFF, 1) == 1.0 - 2.0 ** -25
-    # u24 keeps the top 24 bits: words differing only in the low byte map together
-    assert oracle.lib().qko_uniform(0x12345600, 1) == oracle.lib().qko_uniform(0x123456FF, 1)
+def test_uniform_maps(oracle):
+    u = oracle.lib().qko_uniform
+    # FP64 map: open interval (0, 1), centred cells of width 2^-32
+    assert u(0, 0) == 2.0 ** -33
+    assert u(0xFFFFFFFF, 0) == 1.0 - 2.0 ** -33
+    # FP32 map: (0, 1]; the top words round up to 1
+    assert u(0, 1) == 2.0 ** -33
+    assert u(0xFFFFFFFF, 1) == 1.0
+    # below 2^23 every value is exact in FP32, so the two maps agree
+    for w in (1, 2, 3, 12345, (1 << 23) - 1):
+        assert u(w, 1) == u(w, 0)
+    # round to nearest even at 2^28 (FP32 spacing 32): ties go to the even multiple
+    assert u(0x1234560F, 1) == u(0x12345600, 1) == u(0x12345610, 1)
+    assert u(0x12345611, 1) == u(0x12345620, 1) > u(0x12345600, 1)
+    # against numpy's float32 arithmetic, and within two roundings (half an FP32
+    # ulp of 1 each) of the exact map
+    rng = np.random.default_rng(3)
+    ws = np.concatenate([rng.integers(0, 2 ** 32, 2000, dtype=np.uint64),
+                         np.array([2 ** 24 - 1, 2 ** 24, 2 ** 24 + 1, 2 ** 31, 2 ** 32 - 129, 2 ** 32 - 128],
+                                  dtype=np.uint64)])
+    got = np.array([u(int(w), 1) for w in ws])
+    want = (ws.astype(np.uint32).astype(np.float32) * np.float32(2.0 ** -32) + np.float32(2.0 ** -33))
+    assert np.array_equal(got, want.astype(np.float64))
+    exact = (ws.astype(np.float64) + 0.5) * 2.0 ** -32
+    assert np.max(np.abs(got - exact)) <= 2.0 ** -24
+    order = np.argsort(ws, kind="stable")
+    assert np.all(np.diff(got[order]) >= 0) and got.min() > 0 and got.max() <= 1.0
 
 
 def test_normal_moments(oracle):
