@@ -54,7 +54,13 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
     // registers: the lane block is uniform per wavefront).
     LaneC<Real> cl[L];
 #pragma unroll
-    for (int l = 0; l < L; ++l) cl[l] = lc[l];
+    for (int l = 0; l < L; ++l) {
+        cl[l] = lc[l];
+        // The QE moments m = v e^{-k dt} + tmf and s^2 = v s2f1 + s2f2 take two
+        // uniform operands; with the addends held in vector registers the
+        // step needs no copy of them (one scalar operand per instruction).
+        if constexpr (SCHEME == SCHEME_QE) __asm__("" : "+v"(cl[l].tmf), "+v"(cl[l].s2f2));
+    }
     Real S[L], v[L], mx[L], mn[L];
     double sm[L];
 #pragma unroll
