@@ -165,7 +165,7 @@ static void scheme_init(const qko_model *m, scheme_t *s) {
     }
 }
 
-/* One step of the Heston core.  w are the slot-0 words of (path, step). */
+/* One step of the Heston core.  w are the words of step t (see simulate_one). */
 static void heston_step(const scheme_t *s, const uint32_t w[4], double *S, double *v) {
     double Z1, Z2;
     box_muller(qko_uniform(w[0], s->u24), qko_uniform(w[1], s->u24), &Z1, &Z2);
@@ -188,7 +188,8 @@ static void heston_step(const scheme_t *s, const uint32_t w[4], double *S, doubl
         return;
     }
     /* Andersen QE, MSchemes.hpp:363-441.  The exponential branch draws its
-     * uniform from word 2 instead of Phi(Z_V) (same distribution). */
+     * uniform from the step's third word w[2] instead of Phi(Z_V) (same
+     * distribution). */
     const double Z_V = Z1, Z_indep = Z2;
     const double m = v_n * s->exp_minus_k_dt + s->theta_m_factor;
     const double s2 = v_n * s->s2f1 + s->s2f2;

@@ -442,7 +442,7 @@ __device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real 
         return expo;
     } else {
         // Andersen QE with martingale correction.  The exponential branch draws
-        // its uniform from word 2 of the block (same law as Phi(Z_V)).
+        // its uniform from the step's third word (a2; same law as Phi(Z_V)).
         // FP32: the branches as written; the transcendental unit makes a
         // divergent wavefront's second branch cheap.  FP64: both branches share
         // their (software) transcendentals through lane selects.

@@ -70,6 +70,33 @@ def test_paths_at_counter_offsets_match_oracle(engine, oracle, path0):
     assert np.array_equal(engine.simulate_paths(c, path0=37), whole[:, 37:137])
 
 
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+def test_step_counts_around_block_sharing_match_oracle(engine, oracle, precision):
+    """Heston QE takes four steps from three Philox blocks, Euler/Milstein two
+    steps from one: every step count 1..9 (each remainder of the grouping,
+    with and without whole groups) gives the oracle's paths, through the
+    path-store kernel and through the path-dependent pricing kernel.  FP32
+    paths by the median and the tail as in test_paths_fp32_match_oracle."""
+    pays = c1_payoffs() + [qe.payoff(qk.EUROPEAN, qk.PUT, 1.0)]
+    for scheme in (qk.QE, qk.EULER, qk.MILSTEIN):
+        for n in range(1, 10):
+            c = base_contract(qk.HESTON, scheme, precision, n_paths=1000, n_steps=n, T=n / 52.0)
+            g = engine.simulate_paths(c)
+            o = oracle.simulate(omodel(oracle, c), c.n_paths, store=True)["paths"]
+            assert g.shape == o.shape == (n, c.n_paths)
+            rel = np.abs(g - o) / np.abs(o)
+            if precision == qk.FP64:
+                assert rel.max() < 1e-12, (scheme, n, rel.max())
+                gp = engine.price_payoffs(c, pays)
+                op = oracle.price(omodel(oracle, c), c.n_paths, [opay(oracle, p) for p in pays])
+                for a, b in zip(gp, op):
+                    assert a.itm == b.itm and math.isclose(a.price, b.price, rel_tol=1e-9, abs_tol=1e-12), \
+                        (scheme, n, a.price, b.price)
+            else:
+                assert np.median(rel) < 5e-6, (scheme, n, np.median(rel))
+                assert (rel[-1] > 1e-4).mean() < 0.01, (scheme, n, (rel[-1] > 1e-4).mean())
+
+
 @pytest.mark.parametrize("model,scheme", MODELS)
 def test_paths_fp32_match_oracle(engine, oracle, model, scheme):
     c = base_contract(model, scheme, qk.FP32)
