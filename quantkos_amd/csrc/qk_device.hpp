@@ -450,28 +450,32 @@ __device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real 
         if constexpr (sizeof(Real) == 4) {
             const Real m = v_n * c.ekdt + c.tmf;
             const Real s2 = v_n * c.s2f1 + c.s2f2;
+            // psi = s^2 / m^2 enters only through comparisons and the ratios
+            // 2/psi and (psi -+ 1)/(psi + 1), formed from s^2 and m^2 without
+            // dividing by m.
             if (m <= c.eps) {
                 vn = Real(0);
                 logM = Real(0);
             } else {
-                const Real rm = M::rcp(m);
-                const Real psi = s2 * rm * rm;
-                if (psi <= c.eps) {
+                const Real m2 = m * m;
+                if (s2 <= c.eps * m2) {                          // psi <= eps
                     vn = m;
                     logM = Real(0);
-                } else if (psi <= Real(1.5)) {
-                    const Real two_ip = Real(2) * M::rcp(psi);
+                } else if (s2 <= Real(1.5) * m2) {               // psi <= psi_c
+                    const Real two_ip = (m2 + m2) * M::rcp(s2);  // 2 / psi
                     const Real b2 = two_ip - Real(1) + M::sqrt(two_ip * (two_ip - Real(1)));
                     const Real a = M::div(m, Real(1) + b2);
                     const Real bz = M::sqrt(b2) + z1;
                     vn = a * bz * bz;
-                    const Real safe_A = M::fmin(c.A, Real(0.5) * (Real(1) + b2) * rm - c.eps);
-                    const Real tAa = Real(1) - Real(2) * safe_A * a;
+                    // A clamped to 1/(2a) - eps, applied to the product A a
+                    const Real Aa = M::fmin(c.A * a, Real(0.5) - c.eps * a);
+                    const Real tAa = Real(1) - Real(2) * Aa;
                     // log M = A b^2 a / (1 - 2 A a) - log(1 - 2 A a) / 2
-                    logM = M::div(safe_A * b2 * a, tAa) - M::half_log(tAa);
+                    logM = M::div(Aa * b2, tAa) - M::half_log(tAa);
                 } else {
-                    const Real p = M::div(psi - Real(1), psi + Real(1));
-                    const Real beta = (Real(1) - p) * rm;
+                    const Real rq = M::rcp(s2 + m2);
+                    const Real p = (s2 - m2) * rq;               // (psi - 1) / (psi + 1)
+                    const Real beta = (m + m) * rq;              // (1 - p) / m
                     const Real U = M::uniform(w2);
                     if (U <= p) {
                         vn = Real(0);
@@ -479,8 +483,8 @@ __device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real 
                         const Real safe_U = M::track_min(U, Real(1) - c.eps);   // both positive
                         vn = M::div(M::log(M::div(Real(1) - p, Real(1) - safe_U)), beta);
                     }
-                    const Real safe_A = M::fmin(c.A, beta - c.eps);
-                    logM = M::log(p + M::div(beta * (Real(1) - p), beta - safe_A));
+                    // A clamped to beta - eps: beta - A becomes max(beta - A, eps)
+                    logM = M::log(p + M::div(beta * (Real(1) - p), M::fmax(beta - c.A, c.eps)));
                 }
             }
         } else {
