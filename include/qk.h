@@ -49,7 +49,8 @@ typedef struct qk_ctx qk_ctx;
  * layer applies the reference's Normalize_prices rule (S0 -> 1, K,B /= S0;
  * Config.hpp:710-723) and passes the scale in price_scale, by which every
  * price and SE is multiplied (except cash-or-nothing binaries, scale 1;
- * OptionPricer.hpp:323-332).  dt = T / n_steps. */
+ * OptionPricer.hpp:323-332).  dt = T / n_steps, 1 <= n_steps <= 2^30,
+ * 1 <= n_paths <= 2^40. */
 typedef struct {
     int32_t model, scheme, precision, n_steps;
     int64_t n_paths;

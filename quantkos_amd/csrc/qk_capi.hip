@@ -84,6 +84,8 @@ qk_status validate_contract(qk_ctx *ctx, const qk_contract *c) {
     if (c->scheme < QK_SCHEME_EULER || c->scheme > QK_SCHEME_QE) return fail(ctx, QK_EINVAL, "contract.scheme out of range");
     if (c->precision != QK_FP32 && c->precision != QK_FP64) return fail(ctx, QK_EINVAL, "contract.precision out of range");
     if (c->n_steps < 1) return fail(ctx, QK_EINVAL, "contract.n_steps must be >= 1");
+    // Heston QE draws words 3t..3t+2 of a 32-bit block counter.
+    if (c->n_steps > (1 << 30)) return fail(ctx, QK_EINVAL, "contract.n_steps must be <= 2^30");
     if (c->n_paths < 1) return fail(ctx, QK_EINVAL, "contract.n_paths must be >= 1");
     if (!(c->T > 0.0) || !std::isfinite(c->T)) return fail(ctx, QK_EINVAL, "contract.T must be positive");
     if (!(c->S0 > 0.0) || !std::isfinite(c->S0)) return fail(ctx, QK_EINVAL, "contract.S0 must be positive");

@@ -476,3 +476,22 @@ def test_gpu_normal_moments(engine):
     Z = (np.diff(logS, axis=0) - (r - 0.5 * v0) * dt) / math.sqrt(v0 * dt)
     assert abs(Z.mean()) < 0.005 and abs(Z.var() - 1.0) < 0.01, (Z.mean(), Z.var())
     assert abs(np.corrcoef(Z[:-1].ravel(), Z[1:].ravel())[0, 1]) < 0.005
+
+
+def test_invalid_contracts_are_rejected(engine):
+    """The C ABI's validation: EINVAL with a message, no launch, and the ctx
+    still prices afterwards.  n_steps is capped at 2^30 (the QE step draws
+    words 3t..3t+2 of a 32-bit block counter)."""
+    from quantkos_amd.engine import QKError
+    pays = [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0)]
+    bad = [dict(n_steps=0), dict(n_steps=(1 << 30) + 1), dict(n_paths=0), dict(n_paths=(1 << 40) + 1),
+           dict(rho=1.5), dict(T=0.0), dict(S0=-1.0), dict(precision=7), dict(scheme=3), dict(model=2)]
+    for kw in bad:
+        d = dict(model=qk.HESTON, scheme=qk.QE, precision=qk.FP32)
+        d.update(kw)
+        c = base_contract(d.pop("model"), d.pop("scheme"), d.pop("precision"), **d)
+        with pytest.raises(QKError) as e:
+            engine.price_payoffs(c, pays)
+        assert e.value.status == qk._lib.QK_EINVAL, kw
+    ok = engine.price_payoffs(base_contract(qk.HESTON, qk.QE, qk.FP32), pays)[0]
+    assert ok.n == 1000 and math.isfinite(ok.price)
