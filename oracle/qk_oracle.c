@@ -307,7 +307,7 @@ static void simulate_one(const scheme_t *s, double S0, double v0, uint64_t path,
 
 int qko_simulate(const qko_model *m, int64_t path0, int64_t n_paths, double *S_store,
                  double *S_T, double *v_T, double *S_sum, double *S_max, double *S_min) {
-    if (!m || n_paths < 0 || m->n_steps < 1) return -1;
+    if (!m || n_paths < 0 || m->n_steps < 1 || m->n_steps > (1 << 30)) return -1;
     scheme_t s;
     scheme_init(m, &s);
 #pragma omp parallel for schedule(static)
@@ -395,7 +395,7 @@ static void finish(qko_stats *o, int64_t n, double mean, double m2, int64_t itm,
 
 int qko_price(const qko_model *m, int64_t n_paths, const qko_payoff *pay, int32_t n_pay,
               double price_scale, qko_stats *out) {
-    if (!m || !pay || !out || n_paths < 1 || n_pay < 1 || m->n_steps < 1) return -1;
+    if (!m || !pay || !out || n_paths < 1 || n_pay < 1 || m->n_steps < 1 || m->n_steps > (1 << 30)) return -1;
     double *ST = malloc(sizeof(double) * n_paths * 4);
     double *x = malloc(sizeof(double) * n_paths);
     if (!ST || !x) { free(ST); free(x); return -2; }
@@ -492,7 +492,7 @@ int qko_price_american(const qko_model *m, int64_t n_paths, int32_t right, doubl
 int qko_price_american_ex(const qko_model *m, int64_t n_paths, int32_t right, double K, int32_t nb,
                           int32_t weighted, double price_scale, qko_stats *out, double *coeffs_out,
                           double *cf_out) {
-    if (!m || !out || n_paths < 1 || m->n_steps < 2 || nb < 3 || nb > QKO_MAX_BASIS) return -1;
+    if (!m || !out || n_paths < 1 || m->n_steps < 2 || m->n_steps > (1 << 30) || nb < 3 || nb > QKO_MAX_BASIS) return -1;
     const int32_t N = m->n_steps;
     double *S = malloc(sizeof(double) * n_paths * (size_t)N);
     double *cf = malloc(sizeof(double) * n_paths);
