@@ -492,42 +492,45 @@ __device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real 
             // arguments, so a wavefront whose lanes split between the branches
             // (nearly all of them when v often nears zero) issues each
             // transcendental once instead of twice:
-            //   quadratic (psi <= 1.5):  x = 2/psi, b^2 = x - 1 + sqrt(x(x-1)),
+            //   quadratic (psi <= 1.5):  x = 2/psi = 2m^2/s^2, b^2 = x - 1 + sqrt(x(x-1)),
             //     a = m/(1+b^2), v' = a (b + Z_V)^2,
-            //     log M = A b^2 a/(1-2Aa) - log(1-2Aa)/2          (A clamped to 1/(2a) - eps)
-            //   exponential:  p = (psi-1)/(psi+1), beta = (1-p)/m, 1/beta = m (psi+1)/2,
+            //     log M = A b^2 a/(1-2Aa) - log(1-2Aa)/2  (A a clamped to 1/2 - eps a)
+            //   exponential:  p = (psi-1)/(psi+1) = (s^2-m^2)/(s^2+m^2),
+            //     beta = (1-p)/m = 2m/(s^2+m^2),
             //     v' = 0 if U <= p else log((1-p)/(1-U)) / beta,
             //     M = p + beta (1-p) / max(beta - A, eps)          (A clamped to beta - eps)
             //   psi <= eps: v' = m, M = 1;  m <= eps: v' = 0, M = 1.
+            // Three reciprocals per lane: s^2 or s^2+m^2; 1+b^2 or (1-U) beta,
+            // which gives both 1/(1-U) = beta r2 and 1/beta = (1-U) r2; 1-2Aa
+            // or max(beta - A, eps).  psi itself is never formed.
             const Real m = v_n * c.ekdt + c.tmf;
             const Real s2 = v_n * c.s2f1 + c.s2f2;
             const bool mpos = m > c.eps;
             const Real ms = mpos ? m : Real(1);
-            const Real rm = M::rcp(ms);
-            const Real psi = s2 * rm * rm;
-            const bool quad = psi <= Real(1.5);
-            const Real r1 = M::rcp(quad ? psi : psi + Real(1));
-            const Real p = (psi - Real(1)) * r1;                     // exponential
+            const Real m2 = ms * ms;
+            const bool quad = s2 <= Real(1.5) * m2;
+            const Real r1 = M::rcp(quad ? s2 : s2 + m2);
+            const Real p = (s2 - m2) * r1;                           // exponential
             const Real omp = Real(1) - p;
-            const Real x = Real(2) * r1;                             // quadratic
+            const Real beta = (ms + ms) * r1;
+            const Real x = (m2 + m2) * r1;                           // quadratic
             const Real b2 = x - Real(1) + M::sqrt(quad ? x * (x - Real(1)) : Real(0));
             const Real U = M::uniform(w2);
-            const Real safe_U = M::fmin(U, Real(1) - c.eps);
-            const Real r2 = M::rcp(quad ? Real(1) + b2 : Real(1) - safe_U);
+            const Real omU = Real(1) - M::fmin(U, Real(1) - c.eps);
+            const Real r2 = M::rcp(quad ? Real(1) + b2 : omU * beta);
             const Real a = ms * r2;
             const Real bz = M::sqrt(quad ? b2 : Real(0)) + z1;
-            const Real beta = omp * rm;
-            const Real sAq = M::fmin(c.A, Real(0.5) * (Real(1) + b2) * rm - c.eps);
-            const Real tAa = Real(1) - Real(2) * sAq * a;
+            const Real Aa = M::fmin(c.A * a, Real(0.5) - c.eps * a);
+            const Real tAa = Real(1) - Real(2) * Aa;
             const Real r3 = M::rcp(quad ? tAa : M::fmax(beta - c.A, c.eps));
-            const Real L1 = M::log(quad ? tAa : omp * r2);
+            const Real L1 = M::log(quad ? tAa : omp * (beta * r2));
             Real L2 = Real(0);
             const bool expo_lane = mpos && !quad;
             if (__any(expo_lane)) L2 = M::log(expo_lane ? p + beta * omp * r3 : Real(1));
             const Real vn_quad = a * bz * bz;
-            const Real vn_exp = U <= p ? Real(0) : L1 * (ms * (psi + Real(1)) * Real(0.5));
-            const Real logM_quad = sAq * b2 * a * r3 - Real(0.5) * L1;
-            const bool degenerate = psi <= c.eps;
+            const Real vn_exp = U <= p ? Real(0) : L1 * (omU * r2);
+            const Real logM_quad = Aa * b2 * r3 - Real(0.5) * L1;
+            const bool degenerate = s2 <= c.eps * m2;                 // psi <= eps
             const Real vn_sel = !mpos ? Real(0) : degenerate ? m : quad ? vn_quad : vn_exp;
             const Real logM_sel = (!mpos || degenerate) ? Real(0) : quad ? logM_quad : L2;
             vn = vn_sel;
