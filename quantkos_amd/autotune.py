@@ -13,7 +13,7 @@ Knobs:
   * tile schedule of the request server: path-steps per work item (the tiles a
     wavefront pulls at a time), resident blocks per CU of a batch launch, and
     batch launches in flight.  Chosen by burst throughput first, then by p50
-    and p99 latency under Poisson arrivals among the candidates within 3 % of
+    and p99 latency under Poisson arrivals among the candidates within 5 % of
     the best throughput (larger items raise throughput little but make
     batches coarser, which lengthens the typical request's wait).
 
@@ -43,8 +43,10 @@ LANES_PER_THREAD = (1, 2, 4)
 LANE_COUNTS = (2, 4, 8, 12, 13, 15, 16)
 SERVER_CANDIDATES = [dict(item_path_steps=ips, blocks_per_cu=bpc, max_inflight=mi)
                      for ips in (65536, 131072, 524288) for bpc in (4, 8) for mi in (2, 3, 4)]
-# Burst-throughput band within which the server schedule is chosen by latency.
-SERVER_RATE_TOLERANCE = 0.03
+# Burst-throughput band within which the server schedule is chosen by latency:
+# repeated burst measurements of one schedule differ by up to ~3 %.
+SERVER_RATE_TOLERANCE = 0.05
+SERVER_REPEATS = 2
 
 
 def load(path: str = DEFAULT_PATH):
@@ -145,12 +147,13 @@ def pick_server(rates: dict, latencies: dict, tolerance: float = SERVER_RATE_TOL
     return pick_best({k: math.sqrt(latencies[k][0] * latencies[k][1]) for k in timed})
 
 
-def tune_server(engine, n_requests=384, candidates=None, latency_requests=1024, log=print):
-    """Server schedule: burst throughput (path-steps/s) of the first
-    `n_requests` requests of the config-4 stream (FP32) under each candidate,
-    then p50/p99 latency of a `latency_requests` stream with Poisson arrivals
-    at 0.8x the best burst capacity under each candidate within
-    SERVER_RATE_TOLERANCE of that capacity (pick_server chooses)."""
+def tune_server(engine, n_requests=384, candidates=None, latency_requests=1024, repeats=SERVER_REPEATS,
+                log=print):
+    """Server schedule: burst throughput (path-steps/s, best of `repeats`
+    runs) of the first `n_requests` requests of the config-4 stream (FP32)
+    under each candidate, then p50/p99 latency of a `latency_requests` stream
+    with Poisson arrivals at 0.8x the best burst capacity under each candidate
+    within SERVER_RATE_TOLERANCE of that capacity (pick_server chooses)."""
     from .engine import Server
     from . import workloads
     cs, ps, _ = workloads.config4_stream(n=n_requests, seed=2024, precision=_lib.FP32)
@@ -161,13 +164,14 @@ def tune_server(engine, n_requests=384, candidates=None, latency_requests=1024, 
             srv.replay(cs, ps, [0.0] * len(cs))
         rates = {}
         for cand in candidates or SERVER_CANDIDATES:
-            with Server(engine, **cand) as srv:
-                t0 = time.perf_counter()
-                srv.replay(cs, ps, [0.0] * len(cs))
-                wall = time.perf_counter() - t0
             key = tuple(sorted(cand.items()))
-            rates[key] = total / wall
-            log("server %s -> %.3e path-steps/s" % (cand, total / wall))
+            for _ in range(repeats):                                  # best of the repeats
+                with Server(engine, **cand) as srv:
+                    t0 = time.perf_counter()
+                    srv.replay(cs, ps, [0.0] * len(cs))
+                    wall = time.perf_counter() - t0
+                rates[key] = max(rates.get(key, 0.0), total / wall)
+            log("server %s -> %.3e path-steps/s" % (cand, rates[key]))
         latencies = {}
         if latency_requests:
             top = max(rates.values())
@@ -211,7 +215,7 @@ def autotune(engine, out: str = DEFAULT_PATH, quick: bool = False, log=print) ->
         lanes = tune_lanes(engine, precisions=(_lib.FP32,), schemes=(_lib.MILSTEIN,), lane_counts=(4, 13),
                            n_paths=1 << 16, n_steps=32, repeats=1, log=log)
         server, rates, lat = tune_server(engine, n_requests=64, candidates=SERVER_CANDIDATES[:2],
-                                         latency_requests=64, log=log)
+                                         latency_requests=64, repeats=1, log=log)
     else:
         lanes = tune_lanes(engine, log=log)
         server, rates, lat = tune_server(engine, log=log)
