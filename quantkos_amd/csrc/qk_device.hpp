@@ -497,7 +497,7 @@ __device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real 
             //     log M = A b^2 a/(1-2Aa) - log(1-2Aa)/2  (A a clamped to 1/2 - eps a)
             //   exponential:  p = (psi-1)/(psi+1) = (s^2-m^2)/(s^2+m^2),
             //     beta = (1-p)/m = 2m/(s^2+m^2),
-            //     v' = 0 if U <= p else log((1-p)/(1-U)) / beta,
+            //     v' = 0 if U <= p else log((1-p)/(1-U)) / beta  (= max(0, log(..)/beta)),
             //     M = p + beta (1-p) / max(beta - A, eps)          (A clamped to beta - eps)
             //   psi <= eps: v' = m, M = 1;  m <= eps: v' = 0, M = 1.
             // Three reciprocals per lane: s^2 or s^2+m^2; 1+b^2 or (1-U) beta,
@@ -528,7 +528,8 @@ __device__ __forceinline__ Real heston_step(const LaneC<Real> &c, Real z1, Real 
             const bool expo_lane = mpos && !quad;
             if (__any(expo_lane)) L2 = M::log(expo_lane ? p + beta * omp * r3 : Real(1));
             const Real vn_quad = a * bz * bz;
-            const Real vn_exp = U <= p ? Real(0) : L1 * (omU * r2);
+            // U <= p: the ratio (1-p)/(1-U) is at most 1, L1 <= 0, and the max gives v' = 0
+            const Real vn_exp = M::fmax(L1 * (omU * r2), Real(0));
             const Real logM_quad = Aa * b2 * r3 - Real(0.5) * L1;
             const bool degenerate = s2 <= c.eps * m2;                 // psi <= eps
             const Real vn_sel = !mpos ? Real(0) : degenerate ? m : quad ? vn_quad : vn_exp;
