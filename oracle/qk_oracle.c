@@ -117,7 +117,9 @@ static void scheme_init(const qko_model *m, scheme_t *s) {
         /* Martingale-compensated drift q' = q + lambda*kappa_J (MSchemes.hpp:540-549). */
         const double kappa_J = exp(m->mu_j + 0.5 * m->sigma_j * m->sigma_j) - 1.0;
         q += m->lambda_j * kappa_J;
-        s->exp_minus_lambda_dt = exp(-m->lambda_j * dt);
+        /* lambda = 0: no jump (a threshold above every uniform; the FP32 map
+         * reaches 1, the reference's uniforms stay below it). */
+        s->exp_minus_lambda_dt = m->lambda_j > 0.0 ? exp(-m->lambda_j * dt) : 2.0;
         s->mu_j = m->mu_j;
         s->sigma_j = m->sigma_j;
     }

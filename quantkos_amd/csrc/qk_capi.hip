@@ -125,7 +125,9 @@ LaneC<Real> make_lane(const qk_contract &c) {
     if (c.model == QK_MODEL_BATES) {
         const double kJ = std::exp(c.mu_j + 0.5 * c.sigma_j * c.sigma_j) - 1.0;
         q += c.lambda_j * kJ;
-        eml = std::exp(-c.lambda_j * dt);
+        // lambda = 0 never jumps (the reference's uniforms are below 1; the
+        // FP32 map reaches 1, which would pass a threshold of exactly 1).
+        eml = c.lambda_j > 0.0 ? std::exp(-c.lambda_j * dt) : 2.0;
     }
     const double k = c.kappa, th = c.theta, sg = c.sigma;
     double rho = c.rho;

@@ -495,3 +495,20 @@ def test_invalid_contracts_are_rejected(engine):
         assert e.value.status == qk._lib.QK_EINVAL, kw
     ok = engine.price_payoffs(base_contract(qk.HESTON, qk.QE, qk.FP32), pays)[0]
     assert ok.n == 1000 and math.isfinite(ok.price)
+
+
+@pytest.mark.parametrize("scheme", [qk.EULER, qk.QE])
+def test_zero_jump_intensity_never_jumps(engine, oracle, scheme):
+    """Bates with lambda = 0 on the path whose first Poisson uniform is 1.0 in
+    the FP32 map (seed 42, path 16687499, step 0; found by
+    tests/golden/find_fp32_uniform_one.py): the jump size parameters do not
+    matter, and the path equals the oracle's."""
+    path0 = 16687499
+    def run(**kw):
+        c = base_contract(qk.BATES, scheme, qk.FP32, n_paths=64, n_steps=4, seed=42, **kw)
+        return c, engine.simulate_paths(c, path0=path0)
+    c, a = run(lambda_j=0.0, mu_j=-0.5, sigma_j=0.3)
+    _, b = run(lambda_j=0.0, mu_j=0.0, sigma_j=0.0)
+    assert np.array_equal(a, b)
+    o = oracle.simulate(omodel(oracle, c), c.n_paths, path0=path0, store=True)["paths"]
+    assert np.median(np.abs(a - o) / np.abs(o)) < 5e-6

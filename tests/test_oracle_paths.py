@@ -42,6 +42,35 @@ def test_martingale(oracle, model, scheme):
     assert abs(s.price - math.exp(-0.01)) <= 3.5 * s.se
 
 
+# Seed 42, path 16687499: word 3 of Philox block (path, step 0, slot 0) is at
+# least 2^32 - 128, so the FP32 uniform map gives exactly 1.0 for the step's
+# first Poisson uniform (tests/golden/find_fp32_uniform_one.py finds it).
+U_ONE_SEED, U_ONE_PATH = 42, 16687499
+
+
+@pytest.mark.parametrize("scheme", [0, 2])
+def test_zero_jump_intensity_never_jumps(oracle, scheme):
+    """MSchemes.hpp:502-509: Knuth's loop jumps while the uniform product is >=
+    e^{-lambda dt}; the reference's uniforms are below 1, so lambda = 0 never
+    jumps.  On the path whose first Poisson uniform is 1.0 in the FP32 map, the
+    jump size parameters must not matter at lambda = 0 -- and do at a lambda
+    whose threshold rounds to 1 (the planted failure a threshold of exactly 1
+    would give)."""
+    assert oracle.lib().qko_uniform(oracle.philox((U_ONE_PATH, 0, 0, 0), (U_ONE_SEED, 0))[3], 1) == 1.0
+    base = dict(model=1, scheme=scheme, fp32_semantics=1, n_steps=4, seed=U_ONE_SEED)
+    def paths(**kw):
+        d = dict(base)
+        d.update(kw)
+        return oracle.simulate(M(oracle, **d), 1, path0=U_ONE_PATH, store=True)["paths"]
+    a = paths(lambda_j=0.0, mu_j=-0.5, sigma_j=0.3)
+    b = paths(lambda_j=0.0, mu_j=0.0, sigma_j=0.0)
+    assert np.array_equal(a, b)
+    c = paths(lambda_j=1e-300, mu_j=-0.5, sigma_j=0.3)
+    d = paths(lambda_j=1e-300, mu_j=0.0, sigma_j=0.0)
+    assert not np.array_equal(c, d)
+    assert np.array_equal(b, d)
+
+
 def test_qe_matches_heston_cf_coarse_grid(oracle):
     """Andersen QE is nearly unbiased at coarse grids (Rouah p.28 parameters, N=32)."""
     m = M(oracle, scheme=2, n_steps=32, v0=0.05, r=0.03, q=0.02, T=0.5, kappa=5.0, theta=0.05,
