@@ -113,6 +113,30 @@ qk_status validate_payoff(qk_ctx *ctx, const qk_payoff *p) {
 
 bool path_dependent(int type) { return !(type == QK_EUROPEAN || type == QK_BINARY_CASH || type == QK_BINARY_ASSET); }
 
+// The uniform maps of the kernels, evaluated on the host (the FP32 one in
+// float arithmetic: the product by 2^-32 is exact, the sum rounds to nearest
+// even as on the device).
+float uniform_fp32(uint32_t w) { return (float)w * 0x1.0p-32f + 0x1.0p-33f; }
+double uniform_fp64(uint32_t w) { return ((double)w + 0.5) * 0x1.0p-32; }
+
+// Largest word w* with U(w*) below the Poisson threshold e^{-lambda dt}: both
+// uniform maps are non-decreasing in the word, so U(w) >= threshold exactly
+// when w > w*; 0xFFFFFFFF when no word reaches the threshold (lambda = 0).
+template <typename Real>
+uint32_t jump_threshold_word(Real threshold) {
+    auto U = [](uint32_t w) -> double {
+        if constexpr (sizeof(Real) == 4) return (double)uniform_fp32(w);
+        else return uniform_fp64(w);
+    };
+    if (U(0u) >= (double)threshold) return 0u;   // unreachable: lambda dt <= 8 keeps the threshold above U(0)
+    uint64_t lo = 0, hi = 0x100000000ull;        // U(lo) < threshold; U(hi) >= threshold, hi = 2^32 standing for none
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (U((uint32_t)mid) >= (double)threshold) hi = mid; else lo = mid;
+    }
+    return (uint32_t)lo;
+}
+
 // ---------------------------------------------------------------------------
 // Scheme constants in FP64 (SDEScheme constructors, MSchemes.hpp:104-122,
 // :199-216, :296-352; Bates compensator :535-550).
@@ -179,6 +203,7 @@ LaneC<Real> make_lane(const qk_contract &c) {
     L.eml_dt = (Real)eml;
     L.mu_j = (Real)c.mu_j;
     L.sigma_j = (Real)c.sigma_j;
+    L.jump_w = c.model == QK_MODEL_BATES ? jump_threshold_word<Real>(L.eml_dt) : 0xFFFFFFFFu;
     return L;
 }
 

@@ -83,8 +83,9 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
         for (int l = 0; l < L; ++l) {
             Real inc = heston_step<Real, SCHEME>(cl[l], z1, z2, a2, v[l]);
             if constexpr (MODEL == MODEL_BATES) {
-                const Real u3 = M::uniform(a3);
-                if (u3 >= cl[l].eml_dt) inc += bates_jump_slow<Real>(cl[l], u3, p0, p1, (uint32_t)t, key0, key1);
+                // U(a3) >= e^{-lambda dt} as a compare of the word itself
+                if (a3 > cl[l].jump_w)
+                    inc += bates_jump_slow<Real>(cl[l], M::uniform(a3), p0, p1, (uint32_t)t, key0, key1);
             }
             if constexpr (PATHDEP) {
                 S[l] *= M::exp(inc);

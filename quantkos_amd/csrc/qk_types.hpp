@@ -31,6 +31,7 @@ struct LaneC {
     Real omkdt, ktdt, ssdt, qs2dt, iden;     // Euler / implicit Milstein
     Real ekdt, tmf, s2f1, s2f2, K2, K3, K4, A, cv, eps;   // Andersen QE (cv = -K3/2)
     Real eml_dt, mu_j, sigma_j;      // Bates: exp(-lambda dt), jump mean and vol
+    uint32_t jump_w;                 // Bates: a step jumps iff its Poisson word > jump_w
 };
 
 template <typename Real>
