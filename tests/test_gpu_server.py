@@ -47,6 +47,30 @@ def test_server_matches_plain_launch(engine, precision):
         assert r.latency_s >= r.queue_s >= 0.0
 
 
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+def test_server_step_remainders_match_plain_launch(engine, precision):
+    """The batch kernel's step loops for every remainder of the Philox block
+    sharing (QE: four steps per three blocks; Euler: two steps per block), for
+    path-dependent and European payoffs, against the plain launch."""
+    cs, ps = [], []
+    for i, n in enumerate((1, 2, 3, 5, 6, 7, 50, 101)):
+        for scheme in (qk.QE, qk.EULER):
+            for model in (qk.HESTON, qk.BATES):
+                cs.append(qk.contract(model=model, scheme=scheme, precision=precision, n_steps=n,
+                                      n_paths=3_000 + 64 * i + 7, seed=100 + i, lambda_j=0.5))
+                ps.append(qk.payoff(qk.ASIAN if i % 2 else qk.EUROPEAN, qk.CALL, 1.0))
+    with Server(engine) as srv:
+        res = srv.replay(cs, ps, [0.0] * len(cs))
+    tol = 1e-12 if precision == qk.FP64 else 1e-5
+    for c, p, r in zip(cs, ps, res):
+        assert r.status == 0
+        ref = plain(engine, c, p)
+        assert r.stats.n == c.n_paths, (c.n_steps, c.scheme, c.model)
+        if precision == qk.FP64:
+            assert r.stats.itm == ref.itm, (c.n_steps, c.scheme, c.model)
+        assert math.isclose(r.stats.price, ref.price, rel_tol=tol, abs_tol=1e-9), (c.n_steps, c.scheme, c.model)
+
+
 def test_same_result_alone_and_in_mix(engine):
     cs, ps = small_stream(qk.FP32, n=20, seed=11)
     with Server(engine) as srv:
