@@ -175,13 +175,15 @@ def test_deterministic_and_ragged(engine):
 
 
 @pytest.mark.parametrize("precision", [qk.FP32, qk.FP64])
-def test_lanes_match_single_launches(engine, precision):
-    base = base_contract(qk.HESTON, qk.MILSTEIN, precision, n_paths=30_000, n_steps=50)
+@pytest.mark.parametrize("scheme,n_steps", [(qk.MILSTEIN, 50), (qk.MILSTEIN, 7), (qk.QE, 50), (qk.QE, 7)])
+def test_lanes_match_single_launches(engine, precision, scheme, n_steps):
+    """Six lanes: a group of four (or two) lanes per thread plus the rest,
+    with step counts that leave a remainder of the Philox block sharing."""
     lanes = []
     for i, (S0, v0, r, T) in enumerate([(1.0, 0.04, 0.03, 1.0), (1.01, 0.04, 0.03, 1.0),
                                         (1.0, 0.05, 0.03, 1.0), (1.0, 0.04, 0.031, 1.0),
                                         (1.0, 0.04, 0.03, 0.99), (0.99, 0.03, 0.02, 0.9)]):
-        lanes.append(base_contract(qk.HESTON, qk.MILSTEIN, precision, n_paths=30_000, n_steps=50,
+        lanes.append(base_contract(qk.HESTON, scheme, precision, n_paths=30_000, n_steps=n_steps,
                                    S0=S0, v0=v0, r=r, T=T))
     pays = [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0), qe.payoff(qk.ASIAN, qk.PUT, 1.0)]
     multi = engine.price_lanes(lanes, pays)
