@@ -289,6 +289,7 @@ qk_ctx::~qk_ctx() {
         b->release();
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
+    if (hstage) (void)hipHostFree(hstage);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -324,6 +325,24 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// Pinned staging of at least `bytes` (kept for later calls).
+static qk_status host_stage(qk_ctx *ctx, size_t bytes) {
+    if (bytes <= ctx->hstage_cap) return QK_OK;
+    if (ctx->hstage) (void)hipHostFree(ctx->hstage);
+    ctx->hstage = nullptr;
+    ctx->hstage_cap = 0;
+    const size_t want = std::max<size_t>(bytes, 65536);
+    hipError_t e = hipHostMalloc(&ctx->hstage, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        ctx->hstage = nullptr;
+        return fail(ctx, QK_ENOMEM, std::string("hipHostMalloc of the staging buffer failed: ") + hipGetErrorString(e));
+    }
+    ctx->hstage_cap = want;
+    return QK_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 template <typename Real>
 static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
                            Real *pay_out = nullptr) {
@@ -342,24 +361,33 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     const int64_t n_tiles = (c0.n_paths + TILE - 1) / TILE;
     const size_t groups = (size_t)n_lanes * n_pay;
     qk_status st;
-    if ((st = ctx->lanes.reserve(ctx, hl.size() * sizeof(LaneC<Real>))) != QK_OK) return st;
-    if ((st = ctx->pays.reserve(ctx, hp.size() * sizeof(PayD<Real>))) != QK_OK) return st;
-    if ((st = ctx->disc.reserve(ctx, hdisc.size() * 8)) != QK_OK) return st;
-    if ((st = ctx->scale.reserve(ctx, hscale.size() * 8)) != QK_OK) return st;
+    // Lane constants, payoffs, discounts and scales in one block: one copy.
+    const size_t o_pay = align256(hl.size() * sizeof(LaneC<Real>));
+    const size_t o_disc = o_pay + align256(hp.size() * sizeof(PayD<Real>));
+    const size_t o_scale = o_disc + align256(hdisc.size() * 8);
+    const size_t n_const = o_scale + hscale.size() * 8;
+    // Room for the results too, so that qk_price_lanes reuses the buffer as is.
+    if ((st = host_stage(ctx, std::max(n_const, sizeof(qk_stats) * n_lanes * n_pay))) != QK_OK) return st;
+    if ((st = ctx->lanes.reserve(ctx, n_const)) != QK_OK) return st;
+    // ev[0] follows the last copy out of the staging buffer on the stream.
+    QK_HIP(hipEventSynchronize(ctx->ev[0]));
+    char *hb = (char *)ctx->hstage;
+    std::memcpy(hb, hl.data(), hl.size() * sizeof(LaneC<Real>));
+    std::memcpy(hb + o_pay, hp.data(), hp.size() * sizeof(PayD<Real>));
+    std::memcpy(hb + o_disc, hdisc.data(), hdisc.size() * 8);
+    std::memcpy(hb + o_scale, hscale.data(), hscale.size() * 8);
+    char *db = (char *)ctx->lanes.p;
     if ((st = ctx->psum.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pm2.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pitm.reserve(ctx, groups * n_tiles * 4)) != QK_OK) return st;
     if ((st = ctx->stats.reserve(ctx, groups * sizeof(qk_stats))) != QK_OK) return st;
     if ((st = ctx->mscratch.reserve(ctx, groups * merge_chunks(n_tiles) * MERGE_MOM_BYTES)) != QK_OK) return st;
     hipStream_t s = ctx->stream;
-    QK_HIP(hipMemcpyAsync(ctx->lanes.p, hl.data(), hl.size() * sizeof(LaneC<Real>), hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemcpyAsync(ctx->pays.p, hp.data(), hp.size() * sizeof(PayD<Real>), hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemcpyAsync(ctx->disc.p, hdisc.data(), hdisc.size() * 8, hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemcpyAsync(ctx->scale.p, hscale.data(), hscale.size() * 8, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(db, hb, n_const, hipMemcpyHostToDevice, s));
 
     PathArgs<Real> a{};
-    a.lanes = (const LaneC<Real> *)ctx->lanes.p;
-    a.pays = (const PayD<Real> *)ctx->pays.p;
+    a.lanes = (const LaneC<Real> *)db;
+    a.pays = (const PayD<Real> *)(db + o_pay);
     a.n_lanes = n_lanes;
     a.n_pay = n_pay;
     a.n_steps = c0.n_steps;
@@ -394,7 +422,7 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     }
     QK_HIP(hipEventRecord(ctx->ev[1], s));
     QK_HIP(launch_merge(a.part_sum, a.part_m2, a.part_itm, n_tiles, c0.n_paths, n_lanes, n_pay,
-                        (const double *)ctx->disc.p, (const double *)ctx->scale.p, ctx->mscratch.p,
+                        (const double *)(db + o_disc), (const double *)(db + o_scale), ctx->mscratch.p,
                         (qk_stats *)ctx->stats.p, s));
     QK_HIP(hipEventRecord(ctx->ev[2], s));
     return QK_OK;
@@ -467,8 +495,13 @@ extern "C" qk_status qk_price_lanes(qk_ctx *ctx, const qk_contract *lanes, int32
     st = lanes[0].precision == QK_FP32 ? run_lanes<float>(ctx, lanes, n_lanes, pays, n_pay)
                                        : run_lanes<double>(ctx, lanes, n_lanes, pays, n_pay);
     if (st != QK_OK) return st;
-    QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats) * n_lanes * n_pay, hipMemcpyDeviceToHost, ctx->stream));
+    // Results through the pinned staging (its constants' copy precedes this
+    // one on the stream, so the buffer is free to reuse).
+    const size_t n_out = sizeof(qk_stats) * n_lanes * n_pay;
+    if ((st = host_stage(ctx, n_out)) != QK_OK) return st;
+    QK_HIP(hipMemcpyAsync(ctx->hstage, ctx->stats.p, n_out, hipMemcpyDeviceToHost, ctx->stream));
     QK_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out, ctx->hstage, n_out);
     return finish_timing(ctx, lanes[0].n_paths * (int64_t)lanes[0].n_steps * n_lanes, false);
 }
 

@@ -42,5 +42,9 @@ struct qk_ctx {
     qk::DevBuf payv, hist;   // per-path payoffs and selection histograms of qk_price_distribution
     std::vector<qk_schedule_entry> schedule;   // lanes per thread by (precision, scheme, n_lanes)
     int32_t lsm_mode = QK_LSM_AUTO;             // backward-pass schedule of qk_price_american*
+    // Pinned host staging of a lane call's constants (one host-to-device copy)
+    // and results; every call that uses it synchronises before it returns.
+    void *hstage = nullptr;
+    size_t hstage_cap = 0;
     ~qk_ctx();
 };
