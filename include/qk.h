@@ -107,7 +107,7 @@ size_t qk_footprint_bytes(const qk_contract *c, int32_t n_lanes, int32_t n_payof
 
 /* ---- launch schedule (the autotuner's knob for the lane kernels) ----
  * How many lanes of a qk_price_lanes / qk_price_greeks launch one thread
- * carries, sharing each step's random numbers: 1, 2 or 4.  Entries are keyed
+ * carries, sharing each step's random numbers: 1, 2 or 4, or 8 in FP32.  Entries are keyed
  * by (precision, scheme, n_lanes); a call without a matching entry uses 4 when
  * n_lanes >= 4, else 1.  The schedule changes speed only: prices agree to
  * rounding whatever it is.  Replaces the launch-policy choice the reference

@@ -7,7 +7,8 @@ same prices up to rounding, so tuning changes speed only.
 
 Knobs:
   * lane schedule: how many lanes of a qk_price_lanes / qk_price_greeks launch
-    one thread carries (1, 2 or 4), keyed by (precision, scheme, n_lanes).
+    one thread carries (1, 2 or 4, or 8 in FP32), keyed by (precision, scheme,
+    n_lanes).
     More lanes per thread share one Philox block and one Box-Muller transform
     per step; fewer keep more waves resident and waste no padded lanes.
   * tile schedule of the request server: path-steps per work item (the tiles a
@@ -37,7 +38,7 @@ TUNING_DIR = os.path.join(PKG, "tuning")
 ARCH = os.environ.get("QK_OFFLOAD_ARCH", "gfx950")
 DEFAULT_PATH = os.path.join(TUNING_DIR, ARCH + ".json")
 
-LANES_PER_THREAD = (1, 2, 4)
+LANES_PER_THREAD = (1, 2, 4, 8)      # 8 in FP32 only
 # Lane counts tuned: the Greek stencils (12 lanes, 13 with the central theta
 # lane, 15 with every bump) and generic scenario fans.
 LANE_COUNTS = (2, 4, 8, 12, 13, 15, 16)
@@ -61,8 +62,8 @@ def load(path: str = DEFAULT_PATH):
 
 def validate(table: dict):
     for e in table.get("lanes", []):
-        if e["lanes_per_thread"] not in LANES_PER_THREAD:
-            raise ValueError("lanes_per_thread must be 1, 2 or 4: %r" % (e,))
+        if e["lanes_per_thread"] not in LANES_PER_THREAD or (e["lanes_per_thread"] == 8 and e["precision"] != _lib.FP32):
+            raise ValueError("lanes_per_thread must be 1, 2 or 4, or 8 in FP32: %r" % (e,))
         if not 1 <= e["n_lanes"] <= 64:
             raise ValueError("n_lanes must be in [1, 64]: %r" % (e,))
     srv = table.get("server")
@@ -116,7 +117,7 @@ def tune_lanes(engine, precisions=(_lib.FP32, _lib.FP64), schemes=(_lib.EULER, _
             for n in lane_counts:
                 lanes = _lane_contracts(prec, scheme, n, n_paths, n_steps)
                 times = {}
-                for lpt in LANES_PER_THREAD:
+                for lpt in (l for l in LANES_PER_THREAD if l < 8 or prec == _lib.FP32):
                     engine.set_schedule([(prec, scheme, n, lpt)])
                     engine.price_lanes(lanes, pays)                  # warm-up
                     best = float("inf")

@@ -470,8 +470,9 @@ extern "C" qk_status qk_set_schedule(qk_ctx *ctx, const qk_schedule_entry *entri
     if (n < 0 || n > 256 || (n > 0 && !entries)) return fail(ctx, QK_EINVAL, "schedule: 0 <= n <= 256 entries");
     for (int32_t i = 0; i < n; ++i) {
         const qk_schedule_entry &e = entries[i];
-        if (e.lanes_per_thread != 1 && e.lanes_per_thread != 2 && e.lanes_per_thread != 4)
-            return fail(ctx, QK_EINVAL, "schedule: lanes_per_thread must be 1, 2 or 4");
+        if (e.lanes_per_thread != 1 && e.lanes_per_thread != 2 && e.lanes_per_thread != 4 &&
+            !(e.lanes_per_thread == 8 && e.precision == QK_FP32))
+            return fail(ctx, QK_EINVAL, "schedule: lanes_per_thread must be 1, 2 or 4 (or 8 in FP32)");
         if (e.n_lanes < 1 || e.n_lanes > MAX_LANES) return fail(ctx, QK_EINVAL, "schedule: n_lanes must be in [1, 64]");
     }
     ctx->schedule.assign(entries, entries + n);

@@ -882,6 +882,11 @@ template <typename Real, int MODEL, int SCHEME>
 static void launch_ms(const PathArgs<Real> &a, bool pathdep, bool store, int L, dim3 grid, hipStream_t s) {
     if (store) {
         hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, true, 1>), grid, dim3(BLOCK), 0, s, a);
+    } else if (L == 8 && sizeof(Real) == 4) {   // FP32 only: eight FP64 lanes exceed the register file
+        if constexpr (sizeof(Real) == 4) {
+            if (pathdep) hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, false, 8>), grid, dim3(BLOCK), 0, s, a);
+            else hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, false, false, 8>), grid, dim3(BLOCK), 0, s, a);
+        }
     } else if (L == 4) {
         if (pathdep) hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, true, false, 4>), grid, dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_paths<Real, MODEL, SCHEME, false, false, 4>), grid, dim3(BLOCK), 0, s, a);

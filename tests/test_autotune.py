@@ -61,3 +61,11 @@ def test_pick_server_prefers_latency_within_the_throughput_band():
     assert autotune.pick_server(rates, {}) == a
     # a tighter band excludes b
     assert autotune.pick_server(rates, {a: (20.0, 200.0), b: (9.0, 230.0)}, tolerance=0.01) == a
+
+
+def test_eight_lanes_per_thread_only_in_fp32():
+    ok = {"lanes": [{"precision": _lib.FP32, "scheme": 1, "n_lanes": 13, "lanes_per_thread": 8}], "server": {}}
+    autotune.validate(ok)
+    bad = {"lanes": [{"precision": _lib.FP64, "scheme": 1, "n_lanes": 13, "lanes_per_thread": 8}], "server": {}}
+    with pytest.raises(ValueError):
+        autotune.validate(bad)
