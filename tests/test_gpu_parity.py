@@ -133,6 +133,17 @@ def test_prices_fp64_match_oracle(engine, oracle, model, scheme, right):
         assert math.isclose(gp.se, op.se, rel_tol=1e-8, abs_tol=1e-12), (p.type, gp.se, op.se)
 
 
+@pytest.mark.parametrize("right", [qk.CALL, qk.PUT])
+def test_each_payoff_alone_matches_oracle(engine, oracle, right):
+    """A launch keeps only the running statistics its payoffs read: every
+    payoff type priced on its own equals the oracle (FP64, 1e-9)."""
+    c = base_contract(qk.HESTON, qk.QE, qk.FP64, n_paths=20_011, n_steps=37, price_scale=100.0)
+    for p in all_payoffs(right):
+        g = engine.price_payoffs(c, [p])[0]
+        o = oracle.price(omodel(oracle, c), c.n_paths, [opay(oracle, p)], 100.0)[0]
+        assert g.itm == o.itm and math.isclose(g.price, o.price, rel_tol=1e-9, abs_tol=1e-12), (p.type, g.price, o.price)
+
+
 def c1_payoffs():
     P = qe.payoff
     return [P(qk.ASIAN, qk.CALL, 1.0), P(qk.BARRIER_UP_OUT, qk.CALL, 1.0, 1.3),
