@@ -241,9 +241,17 @@ __device__ __forceinline__ Mom block_tree_merge(Mom m) {
     return Mom{sn[0], smu[0], sq[0], si[0]};
 }
 
+// With one chunk per group (n_tiles <= MERGE_CHUNK) the stage-1 block holds
+// the group's whole merge and finalises it (final != null): the stage-2 merge
+// of one chunk would copy it unchanged.
+__device__ __forceinline__ void finalize_group(const Mom &m, int g, int n_pay, const double *__restrict__ disc,
+                                               const double *__restrict__ scale, qk_stats *__restrict__ out);
+
 __global__ __launch_bounds__(256) void k_merge_chunks(const double *__restrict__ ps, const double *__restrict__ pm2,
                                                       const uint32_t *__restrict__ pitm, int64_t n_tiles,
-                                                      int64_t n_paths, int64_t n_chunks, Mom *__restrict__ out) {
+                                                      int64_t n_paths, int64_t n_chunks, Mom *__restrict__ out,
+                                                      int n_pay, const double *__restrict__ disc,
+                                                      const double *__restrict__ scale, qk_stats *__restrict__ final) {
     const int64_t g = blockIdx.y, chunk = blockIdx.x;
     const int64_t base = g * n_tiles;
     Mom m{0.0, 0.0, 0.0, 0ull};
@@ -259,14 +267,35 @@ __global__ __launch_bounds__(256) void k_merge_chunks(const double *__restrict__
         }
     }
     m = block_tree_merge(m);
-    if (threadIdx.x == 0) out[g * n_chunks + chunk] = m;
+    if (threadIdx.x == 0) {
+        if (final) finalize_group(m, (int)g, n_pay, disc, scale, final);
+        else out[g * n_chunks + chunk] = m;
+    }
+}
+
+// Price, SE and ITM share of group g = lane * n_pay + payoff from its merged
+// moments (OptionPricer.hpp:204-249).
+__device__ __forceinline__ void finalize_group(const Mom &m, int g, int n_pay, const double *__restrict__ disc,
+                                               const double *__restrict__ scale, qk_stats *__restrict__ out) {
+    const int lane = g / n_pay, j = g % n_pay;
+    const double N = m.n;
+    const double var = N > 1.0 ? m.m2 / (N - 1.0) : 0.0;
+    const double f = disc[lane] * scale[j];
+    qk_stats o;
+    o.price = m.mean * f;
+    o.se = sqrt(var / N) * f;
+    o.prob_itm = (double)m.itm / N;
+    o.mean = m.mean;
+    o.m2 = m.m2;
+    o.n = (int64_t)N;
+    o.itm = (int64_t)m.itm;
+    out[g] = o;
 }
 
 __global__ __launch_bounds__(256) void k_merge_final(const Mom *__restrict__ chunks, int64_t n_chunks, int n_pay,
                                                      const double *__restrict__ disc, const double *__restrict__ scale,
                                                      qk_stats *__restrict__ out) {
     const int g = blockIdx.x;
-    const int lane = g / n_pay, j = g % n_pay;
     Mom m{0.0, 0.0, 0.0, 0ull};
     for (int64_t c = threadIdx.x; c < n_chunks; c += 256) {
         const Mom x = chunks[(int64_t)g * n_chunks + c];
@@ -274,20 +303,7 @@ __global__ __launch_bounds__(256) void k_merge_final(const Mom *__restrict__ chu
         m.itm += x.itm;
     }
     m = block_tree_merge(m);
-    if (threadIdx.x == 0) {
-        const double N = m.n;
-        const double var = N > 1.0 ? m.m2 / (N - 1.0) : 0.0;
-        const double f = disc[lane] * scale[j];
-        qk_stats o;
-        o.price = m.mean * f;
-        o.se = sqrt(var / N) * f;
-        o.prob_itm = (double)m.itm / N;
-        o.mean = m.mean;
-        o.m2 = m.m2;
-        o.n = (int64_t)N;
-        o.itm = (int64_t)m.itm;
-        out[g] = o;
-    }
+    if (threadIdx.x == 0) finalize_group(m, g, n_pay, disc, scale, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -923,8 +939,10 @@ hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pit
     const int64_t nc = merge_chunks(n_tiles);
     const int groups = n_lanes * n_pay;
     hipLaunchKernelGGL(k_merge_chunks, dim3((unsigned)nc, (unsigned)groups), dim3(256), 0, s, ps, pm2, pitm, n_tiles,
-                       n_paths, nc, (Mom *)scratch);
-    hipLaunchKernelGGL(k_merge_final, dim3(groups), dim3(256), 0, s, (const Mom *)scratch, nc, n_pay, disc, scale, out);
+                       n_paths, nc, (Mom *)scratch, n_pay, disc, scale, nc == 1 ? out : nullptr);
+    if (nc > 1)
+        hipLaunchKernelGGL(k_merge_final, dim3(groups), dim3(256), 0, s, (const Mom *)scratch, nc, n_pay, disc, scale,
+                           out);
     return hipGetLastError();
 }
 
