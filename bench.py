@@ -86,33 +86,57 @@ def run(engine, precision, steps, warmup):
     return res, dt, kern
 
 
-def cpu_baseline(target_s=15.0):
-    """The FP64 CPU oracle (a port of the reference's per-path algorithm, OpenMP)
-    on a bounded sample of the same workload, on this host's cores."""
+def cpu_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when the host sets it (the
+    GPU box gives one GPU's process a share of 16 cores and sets it to 16),
+    else every core this process may run on."""
     try:
-        cores = len(os.sched_getaffinity(0))
+        avail = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    os.environ["OMP_NUM_THREADS"] = str(cores)
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return (int(env) if env.isdigit() and int(env) > 0 else avail), avail
+
+
+def cpu_baseline(gpu32=None, gpu64=None, windows=5):
+    """The FP64 CPU oracle (a port of the reference's per-path algorithm,
+    OpenMP over paths) on the whole config-1 workload, split into `windows`
+    equal path windows of the same random stream, each timed: the rate is the
+    median window's.  The windows' moments merge into the full-size FP64
+    oracle prices on the GPU's random numbers, against which the GPU's FP64
+    and FP32 prices of the timed run are reported (relative and in SE)."""
+    threads, avail = cpu_threads()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     from oracle import oracle as o
     import quantkos_amd as qk
     c, pays = workload(qk.FP64)
-    mk = lambda n: o.model(model=0, scheme=2, fp32_semantics=0, n_steps=N_STEPS, seed=42, S0=1.0,
-                           v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5,
-                           rho=-0.7, lambda_j=0.0, mu_j=0.0, sigma_j=0.0)
+    m = o.model(model=0, scheme=2, fp32_semantics=0, n_steps=N_STEPS, seed=42, S0=1.0, v0=0.04, r=0.03,
+                q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7, lambda_j=0.0, mu_j=0.0, sigma_j=0.0)
     op = [o.Payoff(p.type, p.right, p.K, p.B) for p in pays]
-    n = 4096 * cores
-    t0 = time.perf_counter()
-    o.price(mk(n), n, op, 100.0)
-    cal = time.perf_counter() - t0
-    n2 = int(max(n, min(4_000_000, n * target_s / max(cal, 1e-3))))
-    t0 = time.perf_counter()
-    o.price(mk(n2), n2, op, 100.0)
-    el = time.perf_counter() - t0
-    return {"value": n2 * N_STEPS / el, "unit": "path-steps/s", "cores": cores, "kind": "port",
-            "sample": "%d paths x %d steps of the same QE workload (3 payoffs), FP64, %.1f s" % (n2, N_STEPS, el)}
+    w = N_PATHS // windows
+    times, parts = [], []
+    for k in range(windows):
+        n_k = w if k < windows - 1 else N_PATHS - w * (windows - 1)
+        t0 = time.perf_counter()
+        parts.append(o.price_window(m, k * w, n_k, op, 100.0))
+        times.append(time.perf_counter() - t0)
+        progress("CPU baseline: window %d/%d, %d paths, %.2f s" % (k + 1, windows, n_k, times[-1]))
+    med = sorted(times)[len(times) // 2]
+    disc = math.exp(-0.03 * 1.0)
+    full = [o.merge_stats([p[j] for p in parts], disc, 100.0) for j in range(len(op))]
+    out = {"value": w * N_STEPS / med, "unit": "path-steps/s", "cores": threads, "kind": "port",
+           "sample": "the whole workload (%d paths x %d steps, 3 payoffs, FP64) in %d windows of %d paths, "
+                     "median window %.2f s (all: %s)" % (N_PATHS, N_STEPS, windows, w, med,
+                                                          ", ".join("%.2f" % t for t in times)),
+           "host_cores_available": avail, "nproc": os.cpu_count(),
+           "oracle_full_size": [{"price": f.price, "se": f.se, "itm": f.itm} for f in full]}
+    if gpu64 is not None:
+        out["gpu_fp64_rel_err_vs_oracle"] = [abs(g.price - f.price) / abs(f.price) for g, f in zip(gpu64, full)]
+        out["gpu_fp64_itm_equal"] = [g.itm == f.itm for g, f in zip(gpu64, full)]
+    if gpu32 is not None:
+        out["gpu_fp32_err_in_se_vs_oracle"] = [(g.price - f.price) / f.se for g, f in zip(gpu32, full)]
+    return out
 
 
 README_PUBLISHED_PRICE, README_PUBLISHED_SE = 5.110195, 0.003025
@@ -372,7 +396,7 @@ def main():
         line["serving"] = serve
     if not args.no_cpu_baseline:
         progress("CPU baseline (oracle, FP64)")
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"] = cpu_baseline(r32, r64)
     print(json.dumps(line))
 
 

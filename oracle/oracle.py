@@ -79,6 +79,7 @@ def lib():
         L.qko_payoff_eval.argtypes = [P(Payoff), d, d, d, d, C.c_int32]
         L.qko_payoff_eval.restype = d
         L.qko_price.argtypes = [P(Model), C.c_int64, P(Payoff), C.c_int32, d, P(Stats)]
+        L.qko_price_window.argtypes = [P(Model), C.c_int64, C.c_int64, P(Payoff), C.c_int32, d, P(Stats)]
         L.qko_price_american.argtypes = [P(Model), C.c_int64, C.c_int32, d, d, P(Stats), P(d), P(d)]
         L.qko_greeks_eval.argtypes = [P(Model), C.c_int64, P(Payoff), P(Bumps), d, P(Greeks)]
         L.qko_chan_merge.argtypes = [P(d), P(d), P(d), d, d, d]
@@ -155,6 +156,32 @@ def price(m: Model, n_paths: int, payoffs, price_scale: float = 1.0):
     if rc != 0:
         raise ValueError("qko_price failed")
     return list(out)
+
+
+def price_window(m: Model, path0: int, n_paths: int, payoffs, price_scale: float = 1.0):
+    """price() over the paths [path0, path0 + n_paths) of the stream."""
+    arr = (Payoff * len(payoffs))(*payoffs)
+    out = (Stats * len(payoffs))()
+    rc = lib().qko_price_window(C.byref(m), path0, n_paths, arr, len(payoffs), price_scale, out)
+    if rc != 0:
+        raise ValueError("qko_price_window failed")
+    return list(out)
+
+
+def merge_stats(parts, disc: float, scale: float) -> Stats:
+    """Chan's merge of per-window (n, mean, M2, itm) in window order, then the
+    pricer's finalisation (OptionPricer.hpp:152-172, :204-249)."""
+    n, mean, m2, itm = 0.0, 0.0, 0.0, 0
+    for s in parts:
+        n, mean, m2 = chan_merge((n, mean, m2), (float(s.n), s.mean, s.m2))
+        itm += s.itm
+    out = Stats()
+    var = m2 / (n - 1.0) if n > 1 else 0.0
+    out.price = mean * disc * scale
+    out.se = (var / n) ** 0.5 * disc * scale
+    out.prob_itm = itm / n
+    out.mean, out.m2, out.n, out.itm = mean, m2, int(n), itm
+    return out
 
 
 def price_american(m: Model, n_paths: int, right: int, K: float, price_scale: float = 1.0,

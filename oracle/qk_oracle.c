@@ -397,12 +397,18 @@ static void finish(qko_stats *o, int64_t n, double mean, double m2, int64_t itm,
 
 int qko_price(const qko_model *m, int64_t n_paths, const qko_payoff *pay, int32_t n_pay,
               double price_scale, qko_stats *out) {
-    if (!m || !pay || !out || n_paths < 1 || n_pay < 1 || m->n_steps < 1 || m->n_steps > (1 << 30)) return -1;
+    return qko_price_window(m, 0, n_paths, pay, n_pay, price_scale, out);
+}
+
+int qko_price_window(const qko_model *m, int64_t path0, int64_t n_paths, const qko_payoff *pay, int32_t n_pay,
+                     double price_scale, qko_stats *out) {
+    if (!m || !pay || !out || path0 < 0 || n_paths < 1 || n_pay < 1 || m->n_steps < 1 || m->n_steps > (1 << 30))
+        return -1;
     double *ST = malloc(sizeof(double) * n_paths * 4);
     double *x = malloc(sizeof(double) * n_paths);
     if (!ST || !x) { free(ST); free(x); return -2; }
     double *sum = ST + n_paths, *mx = ST + 2 * n_paths, *mn = ST + 3 * n_paths;
-    qko_simulate(m, 0, n_paths, NULL, ST, NULL, sum, mx, mn);
+    qko_simulate(m, path0, n_paths, NULL, ST, NULL, sum, mx, mn);
     const double eps = m->fp32_semantics ? 1e-6 : 1e-12;      /* OptionPricer.hpp:316 */
     for (int32_t j = 0; j < n_pay; ++j) {
         for (int64_t i = 0; i < n_paths; ++i)

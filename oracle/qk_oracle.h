@@ -100,6 +100,9 @@ double qko_payoff_eval(const qko_payoff *p, double S_T, double S_sum, double S_m
 /* ---- pricing ---- */
 int qko_price(const qko_model *m, int64_t n_paths, const qko_payoff *pay, int32_t n_pay,
               double price_scale, qko_stats *out);
+/* qko_price over the paths [path0, path0 + n_paths) of the stream. */
+int qko_price_window(const qko_model *m, int64_t path0, int64_t n_paths, const qko_payoff *pay, int32_t n_pay,
+                     double price_scale, qko_stats *out);
 int qko_price_american(const qko_model *m, int64_t n_paths, int32_t right, double K,
                        double price_scale, qko_stats *out, double *coeffs_out, double *cf_out);
 /* nb = 3, 4, 5 basis functions (LaguerreP02/P03/P04); weighted: times exp(-X/2).

@@ -130,7 +130,11 @@ def tune_lanes(engine, precisions=(_lib.FP32, _lib.FP64), schemes=(_lib.EULER, _
                                 "kernel_ms": {str(k): round(v, 4) for k, v in times.items()}})
                 log("lanes prec=%d scheme=%d n_lanes=%d -> %d per thread  %s"
                     % (prec, scheme, n, lpt, " ".join("%d:%.3fms" % kv for kv in times.items())))
-    engine.set_schedule([])
+    # leave the engine's own schedule as it was
+    if engine.tuning:
+        apply(engine, engine.tuning)
+    else:
+        engine.set_schedule([])
     return entries
 
 

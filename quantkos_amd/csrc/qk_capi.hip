@@ -52,7 +52,7 @@ qk_status hip_fail(qk_ctx *ctx, hipError_t e, const char *what) {
     } while (0)
 
 qk_status DevBuf::reserve(qk_ctx *ctx, size_t bytes) {
-    if (bytes <= cap) return QK_OK;
+    if (p != nullptr && bytes <= cap) return QK_OK;   // never OK without memory, even for 0 bytes
     if (p) {
         hipError_t e = hipFree(p);
         p = nullptr;
@@ -98,7 +98,11 @@ qk_status validate_contract(qk_ctx *ctx, const qk_contract *c) {
     if (c->model == QK_MODEL_BATES && c->lambda_j * (c->T / c->n_steps) > 8.0)
         return fail(ctx, QK_EINVAL, "Bates lambda_j * dt must be <= 8");
     if (!(c->price_scale > 0.0)) return fail(ctx, QK_EINVAL, "contract.price_scale must be positive");
-    if (c->n_paths > ((int64_t)1 << 40)) return fail(ctx, QK_EINVAL, "contract.n_paths too large");
+    if (c->n_paths > ((int64_t)1 << 40)) return fail(ctx, QK_EINVAL, "contract.n_paths must be <= 2^40");
+    // Path-step counts, path-store bytes (x 8) and lane products (x 64) stay
+    // far inside 64 bits below this bound.
+    if (c->n_paths > (MAX_PATH_STEPS / c->n_steps))
+        return fail(ctx, QK_EINVAL, "contract.n_paths * n_steps must be <= 2^56");
     return QK_OK;
 }
 
@@ -213,21 +217,33 @@ template LaneC<double> make_lane<double>(const qk_contract &);
 // ---------------------------------------------------------------------------
 // Footprints (device bytes of one call), the budget test of admission.
 // ---------------------------------------------------------------------------
+// Saturating size arithmetic: a footprint that does not fit in size_t is
+// SIZE_MAX, which no budget admits.
+static size_t sat_mul(size_t a, size_t b) {
+    size_t r;
+    return __builtin_mul_overflow(a, b, &r) ? SIZE_MAX : r;
+}
+static size_t sat_add(size_t a, size_t b) {
+    size_t r;
+    return __builtin_add_overflow(a, b, &r) ? SIZE_MAX : r;
+}
+
 size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
+    if (c.n_paths < 1 || c.n_steps < 1) return SIZE_MAX;
     const size_t rs = c.precision == QK_FP32 ? 4 : 8;
     const size_t n_tiles = (size_t)((c.n_paths + TILE - 1) / TILE);
-    const size_t groups = (size_t)n_lanes * std::max(n_pay, 1);
-    size_t b = groups * n_tiles * (8 + 8 + 4);          // tile moments
-    b += (size_t)(n_lanes + 4) * sizeof(LaneC<double>) + MAX_PAYOFFS * sizeof(PayD<double>);
-    b += groups * (sizeof(qk_stats) + 16) + 4096;
-    b += groups * (size_t)merge_chunks((int64_t)n_tiles) * MERGE_MOM_BYTES;
+    const size_t groups = (size_t)std::max(n_lanes, 1) * std::max(n_pay, 1);
+    size_t b = sat_mul(sat_mul(groups, n_tiles), 8 + 8 + 4);          // tile moments
+    b = sat_add(b, (size_t)(n_lanes + 4) * sizeof(LaneC<double>) + MAX_PAYOFFS * sizeof(PayD<double>));
+    b = sat_add(b, groups * (sizeof(qk_stats) + 16) + 4096);
+    b = sat_add(b, sat_mul(groups, sat_mul((size_t)merge_chunks((int64_t)n_tiles), MERGE_MOM_BYTES)));
     if (american) {
-        b += (size_t)c.n_paths * (size_t)c.n_steps * rs;   // path store [date][path]
-        b += (size_t)c.n_paths * rs;                         // cash flows
+        b = sat_add(b, sat_mul(sat_mul((size_t)c.n_paths, (size_t)c.n_steps), rs));   // path store [date][path]
+        b = sat_add(b, sat_mul((size_t)c.n_paths, rs));                                // cash flows
         int64_t chunk = 0;
         // Gram partials: the smaller block size gives the larger block count.
-        b += (size_t)c.n_steps * LSM_MAX_BASIS * 8 +
-             (size_t)2 * pump_blocks(c.n_paths, LSM_SHARED_GROUP, PUMP_MAX_BLOCKS, &chunk) * LSM_STRIDE * 8;
+        b = sat_add(b, sat_mul((size_t)c.n_steps, LSM_MAX_BASIS * 8));
+        b = sat_add(b, (size_t)2 * pump_blocks(c.n_paths, LSM_SHARED_GROUP, PUMP_MAX_BLOCKS, &chunk) * LSM_STRIDE * 8);
     }
     return b;
 }
@@ -343,51 +359,63 @@ static qk_status host_stage(qk_ctx *ctx, size_t bytes) {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Byte layout of a lane call's constants block: lane constants, payoffs,
+// per-lane discounts, per-payoff price scales (each part 256-byte aligned).
+struct ConstLayout {
+    size_t o_pay, o_disc, o_scale, bytes;
+};
 template <typename Real>
-static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
-                           Real *pay_out = nullptr) {
+static ConstLayout const_layout(int n_lanes, int n_pay) {
+    ConstLayout l;
+    l.o_pay = align256((size_t)n_lanes * sizeof(LaneC<Real>));
+    l.o_disc = l.o_pay + align256((size_t)n_pay * sizeof(PayD<Real>));
+    l.o_scale = l.o_disc + align256((size_t)n_lanes * 8);
+    l.bytes = l.o_scale + (size_t)n_pay * 8;
+    return l;
+}
+
+// Writes the constants block of a lane call into host memory hb (at least
+// const_layout(...).bytes long): scheme constants precomputed in FP64, the
+// payoffs, e^{-rT} per lane and the price scale per payoff (1 for cash
+// binaries, OptionPricer.hpp:323-332).
+template <typename Real>
+static void pack_consts(const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
+                        const ConstLayout &lay, char *hb) {
+    LaneC<Real> *hl = (LaneC<Real> *)hb;
+    PayD<Real> *hp = (PayD<Real> *)(hb + lay.o_pay);
+    double *hdisc = (double *)(hb + lay.o_disc), *hscale = (double *)(hb + lay.o_scale);
+    for (int i = 0; i < n_lanes; ++i) {
+        hl[i] = make_lane<Real>(lanes[i]);
+        hdisc[i] = std::exp(-lanes[i].r * lanes[i].T);
+    }
+    for (int j = 0; j < n_pay; ++j) {
+        hp[j] = PayD<Real>{pays[j].type, pays[j].right, (Real)pays[j].K, (Real)pays[j].B};
+        hscale[j] = pays[j].type == QK_BINARY_CASH ? 1.0 : lanes[0].price_scale;
+    }
+}
+
+// Device buffers of one lane call.
+struct LaneBufs {
+    char *consts;          // the constants block (ConstLayout)
+    double *psum, *pm2;    // tile moments [(lane * n_pay + j) * n_tiles + tile]
+    uint32_t *pitm;
+    void *mscratch;        // first-stage merge results
+    qk_stats *stats;       // [n_lanes * n_pay]
+};
+
+// Launches the path kernel(s) and the merge of a lane call on ctx->stream,
+// once the constants block is on the device (or its copy is queued before).
+template <typename Real>
+static qk_status launch_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
+                              const ConstLayout &lay, const LaneBufs &B, Real *pay_out) {
     const qk_contract &c0 = lanes[0];
     const bool pathdep = std::any_of(pays, pays + n_pay, [](const qk_payoff &p) { return path_dependent(p.type); });
     const int L = qk_lanes_per_thread(ctx, c0.precision, c0.scheme, n_lanes);
-    std::vector<LaneC<Real>> hl(n_lanes);
-    std::vector<double> hdisc(n_lanes), hscale(n_pay);
-    for (int i = 0; i < n_lanes; ++i) hl[i] = make_lane<Real>(lanes[i]);
-    for (int i = 0; i < n_lanes; ++i) hdisc[i] = std::exp(-lanes[i].r * lanes[i].T);
-    std::vector<PayD<Real>> hp(n_pay);
-    for (int j = 0; j < n_pay; ++j) {
-        hp[j] = PayD<Real>{pays[j].type, pays[j].right, (Real)pays[j].K, (Real)pays[j].B};
-        hscale[j] = pays[j].type == QK_BINARY_CASH ? 1.0 : c0.price_scale;
-    }
     const int64_t n_tiles = (c0.n_paths + TILE - 1) / TILE;
-    const size_t groups = (size_t)n_lanes * n_pay;
-    qk_status st;
-    // Lane constants, payoffs, discounts and scales in one block: one copy.
-    const size_t o_pay = align256(hl.size() * sizeof(LaneC<Real>));
-    const size_t o_disc = o_pay + align256(hp.size() * sizeof(PayD<Real>));
-    const size_t o_scale = o_disc + align256(hdisc.size() * 8);
-    const size_t n_const = o_scale + hscale.size() * 8;
-    // Room for the results too, so that qk_price_lanes reuses the buffer as is.
-    if ((st = host_stage(ctx, std::max(n_const, sizeof(qk_stats) * n_lanes * n_pay))) != QK_OK) return st;
-    if ((st = ctx->lanes.reserve(ctx, n_const)) != QK_OK) return st;
-    // ev[0] follows the last copy out of the staging buffer on the stream.
-    QK_HIP(hipEventSynchronize(ctx->ev[0]));
-    char *hb = (char *)ctx->hstage;
-    std::memcpy(hb, hl.data(), hl.size() * sizeof(LaneC<Real>));
-    std::memcpy(hb + o_pay, hp.data(), hp.size() * sizeof(PayD<Real>));
-    std::memcpy(hb + o_disc, hdisc.data(), hdisc.size() * 8);
-    std::memcpy(hb + o_scale, hscale.data(), hscale.size() * 8);
-    char *db = (char *)ctx->lanes.p;
-    if ((st = ctx->psum.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
-    if ((st = ctx->pm2.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
-    if ((st = ctx->pitm.reserve(ctx, groups * n_tiles * 4)) != QK_OK) return st;
-    if ((st = ctx->stats.reserve(ctx, groups * sizeof(qk_stats))) != QK_OK) return st;
-    if ((st = ctx->mscratch.reserve(ctx, groups * merge_chunks(n_tiles) * MERGE_MOM_BYTES)) != QK_OK) return st;
     hipStream_t s = ctx->stream;
-    QK_HIP(hipMemcpyAsync(db, hb, n_const, hipMemcpyHostToDevice, s));
-
     PathArgs<Real> a{};
-    a.lanes = (const LaneC<Real> *)db;
-    a.pays = (const PayD<Real> *)(db + o_pay);
+    a.lanes = (const LaneC<Real> *)B.consts;
+    a.pays = (const PayD<Real> *)(B.consts + lay.o_pay);
     a.n_lanes = n_lanes;
     a.n_pay = n_pay;
     a.n_steps = c0.n_steps;
@@ -397,9 +425,9 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     a.key0 = (uint32_t)c0.seed;
     a.key1 = (uint32_t)(c0.seed >> 32);
     a.itm_eps = (Real)(c0.precision == QK_FP32 ? 1e-6 : 1e-12);   // OptionPricer.hpp:316
-    a.part_sum = (double *)ctx->psum.p;
-    a.part_m2 = (double *)ctx->pm2.p;
-    a.part_itm = (uint32_t *)ctx->pitm.p;
+    a.part_sum = B.psum;
+    a.part_m2 = B.pm2;
+    a.part_itm = B.pitm;
     a.store = nullptr;
     a.pay_out = pay_out;
 
@@ -422,10 +450,37 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     }
     QK_HIP(hipEventRecord(ctx->ev[1], s));
     QK_HIP(launch_merge(a.part_sum, a.part_m2, a.part_itm, n_tiles, c0.n_paths, n_lanes, n_pay,
-                        (const double *)(db + o_disc), (const double *)(db + o_scale), ctx->mscratch.p,
-                        (qk_stats *)ctx->stats.p, s));
+                        (const double *)(B.consts + lay.o_disc), (const double *)(B.consts + lay.o_scale), B.mscratch,
+                        B.stats, s));
     QK_HIP(hipEventRecord(ctx->ev[2], s));
     return QK_OK;
+}
+
+// The allocating lane call: buffers from the ctx (grown as needed), constants
+// through the pinned staging buffer.  Leaves per-(lane, payoff) qk_stats in
+// ctx->stats on the device.
+template <typename Real>
+static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
+                           Real *pay_out = nullptr) {
+    const ConstLayout lay = const_layout<Real>(n_lanes, n_pay);
+    const int64_t n_tiles = (lanes[0].n_paths + TILE - 1) / TILE;
+    const size_t groups = (size_t)n_lanes * n_pay;
+    qk_status st;
+    // Room for the results too, so that qk_price_lanes reuses the buffer as is.
+    if ((st = host_stage(ctx, std::max(lay.bytes, sizeof(qk_stats) * groups))) != QK_OK) return st;
+    if ((st = ctx->lanes.reserve(ctx, lay.bytes)) != QK_OK) return st;
+    if ((st = ctx->psum.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
+    if ((st = ctx->pm2.reserve(ctx, groups * n_tiles * 8)) != QK_OK) return st;
+    if ((st = ctx->pitm.reserve(ctx, groups * n_tiles * 4)) != QK_OK) return st;
+    if ((st = ctx->stats.reserve(ctx, groups * sizeof(qk_stats))) != QK_OK) return st;
+    if ((st = ctx->mscratch.reserve(ctx, groups * merge_chunks(n_tiles) * MERGE_MOM_BYTES)) != QK_OK) return st;
+    // ev[0] follows the last copy out of the staging buffer on the stream.
+    QK_HIP(hipEventSynchronize(ctx->ev[0]));
+    pack_consts<Real>(lanes, n_lanes, pays, n_pay, lay, (char *)ctx->hstage);
+    QK_HIP(hipMemcpyAsync(ctx->lanes.p, ctx->hstage, lay.bytes, hipMemcpyHostToDevice, ctx->stream));
+    const LaneBufs B{(char *)ctx->lanes.p, (double *)ctx->psum.p, (double *)ctx->pm2.p, (uint32_t *)ctx->pitm.p,
+                     ctx->mscratch.p, (qk_stats *)ctx->stats.p};
+    return launch_lanes<Real>(ctx, lanes, n_lanes, pays, n_pay, lay, B, pay_out);
 }
 
 static qk_status finish_timing(qk_ctx *ctx, int64_t path_steps, bool has_lsm) {
@@ -515,6 +570,7 @@ extern "C" qk_status qk_price(qk_ctx *ctx, const qk_contract *c, const qk_payoff
     return qk_price_lanes(ctx, c, 1, p, 1, out);
 }
 
+
 // ---------------------------------------------------------------------------
 // Greeks: the reference's 12-pricing stencil (MCEngine.hpp:576-735) as lanes
 // of one launch under common random numbers, differenced in FP64 on the device.
@@ -527,8 +583,9 @@ extern "C" qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk
     const double vol0 = std::sqrt(c->v0);
     if ((b->delta_gamma || b->vanna) && !(b->rel_S > 0.0 && b->rel_S <= 0.05))
         return fail(ctx, QK_EINVAL, "rel_S must be in (0, 0.05]");
-    if ((b->vega_vomma || b->vanna) && !(b->abs_vol > 0.0 && b->abs_vol < vol0))
-        return fail(ctx, QK_EINVAL, "abs_vol must be in (0, sqrt(v0))");
+    // abs_vol = sqrt(v0) is allowed: the down-bump is v0 = 0 (Config.hpp:518).
+    if ((b->vega_vomma || b->vanna) && !(b->abs_vol > 0.0 && b->abs_vol <= vol0))
+        return fail(ctx, QK_EINVAL, "abs_vol must be in (0, sqrt(v0)]");
     if (b->rho && !(b->abs_r > 0.0 && b->abs_r <= 0.05)) return fail(ctx, QK_EINVAL, "abs_r must be in (0, 0.05]");
     if (b->theta && !(b->abs_T > 0.0 && b->abs_T < c->T)) return fail(ctx, QK_EINVAL, "abs_T must be in (0, T)");
     std::vector<qk_contract> L;

@@ -205,7 +205,7 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
 // order is fixed by the tile index: results do not depend on the grid shape.
 template <typename Real, int MODEL, int SCHEME, bool PATHDEP, bool STORE, int L>
 __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
-    const int64_t tile = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
+    const int64_t tile = a.tile_off + (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
     if (tile >= a.n_tiles) return;                       // uniform per wavefront
     const int lbase = blockIdx.y * L;
     run_tile<Real, MODEL, SCHEME, PATHDEP, STORE, L>(
@@ -915,20 +915,31 @@ static void launch_ms(const PathArgs<Real> &a, bool pathdep, bool store, int L, 
     }
 }
 
+// One launch per window of at most PATHS_WINDOW_BLOCKS blocks (a single
+// launch below 2^30 paths); every window writes its own tiles' moments, so
+// the windows change nothing in the results.
 template <typename Real>
 hipError_t launch_paths(const PathArgs<Real> &a, int model, int scheme, bool pathdep, bool store, int L,
                         hipStream_t s) {
-    const dim3 grid((unsigned)((a.n_tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK), (unsigned)((a.n_lanes + L - 1) / L));
-    switch (model * 3 + scheme) {
-    case 0: launch_ms<Real, MODEL_HESTON, SCHEME_EULER>(a, pathdep, store, L, grid, s); break;
-    case 1: launch_ms<Real, MODEL_HESTON, SCHEME_MILSTEIN>(a, pathdep, store, L, grid, s); break;
-    case 2: launch_ms<Real, MODEL_HESTON, SCHEME_QE>(a, pathdep, store, L, grid, s); break;
-    case 3: launch_ms<Real, MODEL_BATES, SCHEME_EULER>(a, pathdep, store, L, grid, s); break;
-    case 4: launch_ms<Real, MODEL_BATES, SCHEME_MILSTEIN>(a, pathdep, store, L, grid, s); break;
-    case 5: launch_ms<Real, MODEL_BATES, SCHEME_QE>(a, pathdep, store, L, grid, s); break;
-    default: return hipErrorInvalidValue;
+    if (model < 0 || model > 1 || scheme < 0 || scheme > 2) return hipErrorInvalidValue;
+    const int64_t blocks = (a.n_tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    const unsigned gy = (unsigned)((a.n_lanes + L - 1) / L);
+    for (int64_t b0 = 0; b0 < blocks; b0 += PATHS_WINDOW_BLOCKS) {
+        PathArgs<Real> w = a;
+        w.tile_off = b0 * WAVES_PER_BLOCK;
+        const dim3 grid((unsigned)std::min<int64_t>(blocks - b0, PATHS_WINDOW_BLOCKS), gy);
+        switch (model * 3 + scheme) {
+        case 0: launch_ms<Real, MODEL_HESTON, SCHEME_EULER>(w, pathdep, store, L, grid, s); break;
+        case 1: launch_ms<Real, MODEL_HESTON, SCHEME_MILSTEIN>(w, pathdep, store, L, grid, s); break;
+        case 2: launch_ms<Real, MODEL_HESTON, SCHEME_QE>(w, pathdep, store, L, grid, s); break;
+        case 3: launch_ms<Real, MODEL_BATES, SCHEME_EULER>(w, pathdep, store, L, grid, s); break;
+        case 4: launch_ms<Real, MODEL_BATES, SCHEME_MILSTEIN>(w, pathdep, store, L, grid, s); break;
+        default: launch_ms<Real, MODEL_BATES, SCHEME_QE>(w, pathdep, store, L, grid, s); break;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 int64_t merge_chunks(int64_t n_tiles) { return (n_tiles + MERGE_CHUNK - 1) / MERGE_CHUNK; }

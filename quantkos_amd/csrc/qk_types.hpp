@@ -19,6 +19,10 @@ constexpr int BLOCK = TILE * WAVES_PER_BLOCK;
 constexpr int MAX_PAYOFFS = 16;
 constexpr int MAX_LANES = 64;
 constexpr int MERGE_CHUNK = 1024;   // tiles per first-stage merge block
+constexpr int64_t MAX_PATH_STEPS = (int64_t)1 << 56;   // n_paths * n_steps of one contract
+// Blocks of one k_paths launch: HIP caps a launch at 2^32 - 1 work-items, so
+// larger tile counts go in windows of this many blocks (2^24 tiles).
+constexpr int64_t PATHS_WINDOW_BLOCKS = (int64_t)1 << 22;
 
 // Per-lane scheme constants, precomputed on the host in FP64 (the reference's
 // SDEScheme constructors, MSchemes.hpp:104-122, :199-216, :296-352, :485-491)
@@ -47,6 +51,7 @@ struct PathArgs {
     int32_t n_lanes, n_pay, n_steps, pad0;
     int64_t n_paths, n_tiles;
     int64_t path0;              // global index of path 0 (random-stream counter offset)
+    int64_t tile_off;           // tile of this launch's block 0 (windowed launches)
     uint32_t key0, key1;
     Real itm_eps;
     double *part_sum;           // [(lane*n_pay + j) * n_tiles + tile]

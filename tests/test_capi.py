@@ -64,3 +64,21 @@ def test_ctx_create_reports_missing_device_as_error():
         L.qk_ctx_destroy(h)
         pytest.skip("a GPU is present")
     assert st in (-3, -1) and not h.value
+
+
+def test_footprint_saturates_instead_of_wrapping():
+    """A path store of n_paths * n_steps * 8 bytes beyond 2^64 reads as SIZE_MAX
+    (no budget admits it) instead of wrapping to a small size; so do n_paths
+    or n_steps below 1."""
+    from quantkos_amd import _lib
+    import quantkos_amd as qk
+    L = _lib.load()
+    size_max = 2 ** (8 * C.sizeof(C.c_size_t)) - 1
+    c = qk.contract(n_paths=1 << 31, n_steps=1 << 30, precision=qk.FP64)
+    assert L.qk_footprint_bytes(C.byref(c), 1, 1, 1) == size_max
+    c = qk.contract(n_paths=1 << 40, n_steps=1 << 30, precision=qk.FP32)
+    assert L.qk_footprint_bytes(C.byref(c), 1, 1, 1) == size_max
+    c = qk.contract(n_paths=0, n_steps=10)
+    assert L.qk_footprint_bytes(C.byref(c), 1, 1, 0) == size_max
+    c = qk.contract(n_paths=1 << 20, n_steps=100, precision=qk.FP32)
+    assert L.qk_footprint_bytes(C.byref(c), 1, 1, 1) < 2 ** 30

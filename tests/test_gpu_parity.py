@@ -494,10 +494,11 @@ def test_gpu_normal_moments(engine):
 def test_invalid_contracts_are_rejected(engine):
     """The C ABI's validation: EINVAL with a message, no launch, and the ctx
     still prices afterwards.  n_steps is capped at 2^30 (the QE step draws
-    words 3t..3t+2 of a 32-bit block counter)."""
+    words 3t..3t+2 of a 32-bit block counter), n_paths * n_steps at 2^56."""
     from quantkos_amd.engine import QKError
     pays = [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0)]
     bad = [dict(n_steps=0), dict(n_steps=(1 << 30) + 1), dict(n_paths=0), dict(n_paths=(1 << 40) + 1),
+           dict(n_paths=1 << 31, n_steps=1 << 30), dict(n_paths=(1 << 40), n_steps=1 << 17),
            dict(rho=1.5), dict(T=0.0), dict(S0=-1.0), dict(precision=7), dict(scheme=3), dict(model=2)]
     for kw in bad:
         d = dict(model=qk.HESTON, scheme=qk.QE, precision=qk.FP32)

@@ -140,3 +140,63 @@ def test_config4_stream_poisson(engine):
     assert all(math.isfinite(r.stats.price) and r.stats.price >= 0 for r in res)
     lat = sorted(r.latency_s for r in res)
     assert lat[len(lat) // 2] > 0
+
+
+def oracle_model(o, c):
+    return o.model(model=c.model, scheme=c.scheme, fp32_semantics=int(c.precision == qk.FP32),
+                   n_steps=c.n_steps, seed=c.seed, S0=c.S0, v0=c.v0, r=c.r, q=c.q, T=c.T, kappa=c.kappa,
+                   theta=c.theta, sigma=c.sigma, rho=c.rho, lambda_j=c.lambda_j, mu_j=c.mu_j,
+                   sigma_j=c.sigma_j)
+
+
+def test_server_mix_matches_oracle(engine, oracle):
+    """A mixed FP64 batch (every payoff family, Heston and Bates QE, American by
+    Longstaff-Schwartz) against the FP64 oracle on the same random numbers: the
+    reference prices each request by one MCDispatcher run (Distpatcher.hpp:78-83,
+    ForwardMC / ForwardBackwardMC.hpp:204-255).  Non-American prices within 1e-9
+    relative with exact ITM counts; American prices within 0.02 standard errors
+    (the early-date Laguerre regressions are badly conditioned, so summation
+    order moves single coefficients, not the fitted policy)."""
+    cs, ps = small_stream(qk.FP64, n=40, seed=21)
+    with Server(engine) as srv:
+        res = srv.replay(cs, ps, [0.0005 * k for k in range(len(cs))])
+    kinds = set()
+    for c, p, r in zip(cs, ps, res):
+        assert r.status == 0
+        m = oracle_model(oracle, c)
+        kinds.add(p.type)
+        if p.type == qk.AMERICAN:
+            ref = oracle.price_american(m, c.n_paths, p.right, p.K, c.price_scale)
+            assert abs(r.stats.price - ref.price) <= 0.02 * ref.se + 1e-12, (r.stats.price, ref.price, ref.se)
+        else:
+            ref = oracle.price(m, c.n_paths, [oracle.Payoff(p.type, p.right, p.K, p.B)], c.price_scale)[0]
+            assert math.isclose(r.stats.price, ref.price, rel_tol=1e-9, abs_tol=1e-12), (p.type, r.stats.price,
+                                                                                          ref.price)
+            assert r.stats.itm == ref.itm and r.stats.n == ref.n
+            assert math.isclose(r.stats.se, ref.se, rel_tol=1e-7, abs_tol=1e-12)
+    assert qk.AMERICAN in kinds and qk.EUROPEAN in kinds and len(kinds) >= 6, kinds
+
+
+def test_config4_stream_full_size(engine):
+    """The config-4 stream at its size: 4,096 requests, paths log-uniform in
+    1e4..1e7, steps in {12, 52, 252, 504}, the whole mix, burst-submitted to the
+    server in FP32 and again in FP64 (same random numbers).  Every request
+    completes, the server counts exactly the stream's path-steps, and every
+    non-American FP32 price is within 1 standard error of its FP64 price."""
+    cs32, ps, _ = workloads.config4_stream(n=4096, seed=2024, precision=qk.FP32)
+    cs64, _, _ = workloads.config4_stream(n=4096, seed=2024, precision=qk.FP64)
+    out = {}
+    for name, cs in (("fp32", cs32), ("fp64", cs64)):
+        with Server(engine) as srv:
+            res = srv.replay(cs, ps, [0.0] * len(cs))
+            st = srv.stats()
+        assert st.n_completed == len(cs) and st.path_steps == workloads.path_steps(cs), name
+        assert all(r.status == 0 for r in res), name
+        assert all(r.stats.n == c.n_paths for r, c in zip(res, cs)), name
+        out[name] = res
+    worst = 0.0
+    for p, a, b in zip(ps, out["fp32"], out["fp64"]):
+        if p.type == qk.AMERICAN or b.stats.se == 0.0:
+            continue
+        worst = max(worst, abs(a.stats.price - b.stats.price) / b.stats.se)
+    assert worst <= 1.0, worst

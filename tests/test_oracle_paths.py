@@ -288,3 +288,20 @@ def test_american_other_bases_ikonen_toivanen(oracle, nb, weighted):
           theta=0.16, sigma=0.9, rho=0.1)
     s = oracle.price_american(m, 100_000, 1, 1.0, 10.0, n_basis=nb, weighted=weighted)
     assert abs(s.price - 0.520030) <= 3 * s.se + max(0.02 * 0.520030, 0.005), (s.price, s.se)
+
+
+def test_price_windows_merge_to_the_whole(oracle):
+    """Pricing path windows [0, a), [a, b), [b, n) of the stream and merging their
+    moments (Chan, OptionPricer.hpp:152-172) gives the one-shot price: the
+    stream is keyed by the global path index."""
+    m = M(oracle, model=0, scheme=2, n_steps=30, seed=42, S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0,
+          theta=0.04, sigma=0.5, rho=-0.7)
+    pays = [oracle.Payoff(1, 0, 1.0, 1.0), oracle.Payoff(2, 0, 1.0, 1.3), oracle.Payoff(6, 1, 1.0, 1.0)]
+    n, cuts = 20_000, (0, 6_401, 13_000, 20_000)
+    whole = oracle.price(m, n, pays, 100.0)
+    parts = [oracle.price_window(m, a, b - a, pays, 100.0) for a, b in zip(cuts, cuts[1:])]
+    for j, w in enumerate(whole):
+        mg = oracle.merge_stats([p[j] for p in parts], math.exp(-m.r * m.T), 100.0)
+        assert mg.n == w.n and mg.itm == w.itm
+        assert math.isclose(mg.price, w.price, rel_tol=1e-12)
+        assert math.isclose(mg.se, w.se, rel_tol=1e-9)
