@@ -141,6 +141,18 @@ qk_status qk_price(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoff, q
 qk_status qk_price_payoffs(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoffs,
                            int32_t n_payoffs, qk_stats *out);
 
+/* qk_price_payoffs without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc).  Every device buffer of the
+ * call comes from a fixed array of QK_NOALLOC_SCRATCH_BYTES per device,
+ * reserved when the library loads; about 20 bytes per 64 paths per payoff, so
+ * up to ~16M paths with 3 payoffs.  Larger calls fail with QK_ENOMEM and
+ * allocate nothing.  The array is shared by every call on a device: calls
+ * that overlap on one device race.  Results equal qk_price_payoffs'. */
+#define QK_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+qk_status qk_price_payoffs_noalloc(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoffs,
+                                   int32_t n_payoffs, qk_stats *out);
+
 /* Lanes = contracts that share model, scheme, precision, n_steps, n_paths and
  * seed, and therefore the random stream (common random numbers); they may
  * differ in S0, v0, r, q, T and the Heston/jump parameters.  One launch

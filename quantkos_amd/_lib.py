@@ -20,6 +20,7 @@ CALL, PUT = 0, 1
 FP32, FP64 = 0, 1
 LAGUERRE_P02, LAGUERRE_P03, LAGUERRE_P04 = 0, 1, 2
 LSM_AUTO, LSM_PER_DATE, LSM_COOPERATIVE = 0, 1, 2
+NOALLOC_SCRATCH_BYTES = 16 << 20   # QK_NOALLOC_SCRATCH_BYTES
 
 
 class Contract(C.Structure):
@@ -130,6 +131,7 @@ def load():
         "qk_lanes_per_thread": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int32),
         "qk_price": ([vp, P(Contract), P(Payoff), P(Stats)], st),
         "qk_price_payoffs": ([vp, P(Contract), P(Payoff), C.c_int32, P(Stats)], st),
+        "qk_price_payoffs_noalloc": ([vp, P(Contract), P(Payoff), C.c_int32, P(Stats)], st),
         "qk_price_lanes": ([vp, P(Contract), C.c_int32, P(Payoff), C.c_int32, P(Stats)], st),
         "qk_price_greeks": ([vp, P(Contract), P(Payoff), C.c_int32, P(Bumps), P(Greeks)], st),
         "qk_price_american": ([vp, P(Contract), C.c_int32, C.c_double, P(Stats), P(C.c_double)], st),

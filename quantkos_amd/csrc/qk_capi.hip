@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -280,6 +281,7 @@ extern "C" qk_status qk_ctx_create(int device, size_t mem_budget_bytes, qk_ctx *
         return QK_EDEVICE;
     }
     ctx->budget = mem_budget_bytes ? mem_budget_bytes : (size_t)(0.85 * (double)free_b);
+    noalloc_init(device);   // device address of the allocation-free entry point's array
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return QK_EDEVICE;
@@ -570,6 +572,101 @@ extern "C" qk_status qk_price(qk_ctx *ctx, const qk_contract *c, const qk_payoff
     return qk_price_lanes(ctx, c, 1, p, 1, out);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free pricing: qk_price_payoffs over a fixed device array.
+//
+// `scratch` holds every device buffer of the call (constants block, tile
+// moments, merge results, result structs).  Each device that loads the
+// library holds its own copy until the process ends.  In host code the name
+// `scratch` is only a host-side stand-in: kernels get the device address that
+// hipGetSymbolAddress returned for the current device (noalloc_ptr), never the
+// name.  Every call on a device shares that device's array, so calls that
+// overlap on one device (from several threads or several ctxs) race.  Nothing
+// in it needs clearing: the path kernels write every tile moment the merge
+// reads, and the merge writes every result.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[QK_NOALLOC_SCRATCH_BYTES];
+
+namespace qk {
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+void *noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+hipError_t noalloc_err[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+// Looks up the device address of `scratch` on `device`, which must be the
+// current device; once per device (qk_ctx_create calls it).
+void noalloc_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[device] != nullptr) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p != nullptr) noalloc_ptr[device] = p;
+    else noalloc_err[device] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+}
+
+static char *noalloc_base(qk_ctx *ctx, qk_status *st) {
+    const int d = ctx->device;
+    if (d < 0 || d >= NOALLOC_MAX_DEVICES) {
+        *st = fail(ctx, QK_EDEVICE, "no-allocation scratch: device index beyond 63");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[d] == nullptr) {
+        *st = hip_fail(ctx, noalloc_err[d], "hipGetSymbolAddress(scratch)");
+        return nullptr;
+    }
+    *st = QK_OK;
+    return (char *)noalloc_ptr[d];
+}
+}  // namespace qk
+
+extern "C" qk_status qk_price_payoffs_noalloc(qk_ctx *ctx, const qk_contract *c, const qk_payoff *pays,
+                                              int32_t n_pay, qk_stats *out) {
+    if (!ctx || !out) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_lanes(ctx, c, 1, pays, n_pay);
+    if (st != QK_OK) return st;
+    const bool f32 = c->precision == QK_FP32;
+    const ConstLayout lay = f32 ? const_layout<float>(1, n_pay) : const_layout<double>(1, n_pay);
+    // Carve the array: constants, results, tile moments, merge results.
+    const size_t n_tiles = (size_t)((c->n_paths + TILE - 1) / TILE);
+    const size_t groups = (size_t)n_pay;
+    const size_t o_stats = align256(lay.bytes);
+    const size_t o_sum = o_stats + align256(groups * sizeof(qk_stats));
+    const size_t mom = groups * n_tiles;                 // <= 16 * 2^34: no overflow
+    const size_t o_m2 = o_sum + align256(mom * 8);
+    const size_t o_itm = o_m2 + align256(mom * 8);
+    const size_t o_merge = o_itm + align256(mom * 4);
+    const size_t need = o_merge + groups * (size_t)merge_chunks((int64_t)n_tiles) * MERGE_MOM_BYTES;
+    if (need > QK_NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, QK_ENOMEM, "qk_price_payoffs_noalloc needs " + std::to_string(need) +
+                                        " bytes of scratch, the fixed array holds " +
+                                        std::to_string((size_t)QK_NOALLOC_SCRATCH_BYTES) +
+                                        " (about 20 bytes per 64 paths per payoff)");
+    QK_HIP(hipSetDevice(ctx->device));
+    char *base = noalloc_base(ctx, &st);
+    if (st != QK_OK) return st;
+    // The constants block is built on the stack: this call allocates nothing,
+    // on the device or the host.
+    alignas(256) char hb[NOALLOC_CONST_BYTES];
+    if (lay.bytes > sizeof(hb)) return fail(ctx, QK_EINVAL, "constants block too large");
+    if (f32) pack_consts<float>(c, 1, pays, n_pay, lay, hb);
+    else pack_consts<double>(c, 1, pays, n_pay, lay, hb);
+    QK_HIP(hipMemcpyAsync(base, hb, lay.bytes, hipMemcpyHostToDevice, ctx->stream));
+    const LaneBufs B{base, (double *)(base + o_sum), (double *)(base + o_m2), (uint32_t *)(base + o_itm),
+                     base + o_merge, (qk_stats *)(base + o_stats)};
+    st = f32 ? launch_lanes<float>(ctx, c, 1, pays, n_pay, lay, B, nullptr)
+             : launch_lanes<double>(ctx, c, 1, pays, n_pay, lay, B, nullptr);
+    if (st != QK_OK) {
+        (void)hipStreamSynchronize(ctx->stream);   // hb must outlive any queued copy
+        return st;
+    }
+    QK_HIP(hipMemcpyAsync(out, B.stats, groups * sizeof(qk_stats), hipMemcpyDeviceToHost, ctx->stream));
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, false);
+}
 
 // ---------------------------------------------------------------------------
 // Greeks: the reference's 12-pricing stencil (MCEngine.hpp:576-735) as lanes

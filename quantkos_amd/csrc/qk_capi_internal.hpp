@@ -28,6 +28,10 @@ template <typename Real>
 LaneC<Real> make_lane(const qk_contract &c);
 size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american);
 qk_status check_budget(qk_ctx *ctx, size_t bytes);
+void noalloc_init(int device);
+// Host bytes of the constants block of qk_price_payoffs_noalloc (one lane,
+// at most MAX_PAYOFFS payoffs), built on the stack.
+constexpr size_t NOALLOC_CONST_BYTES = 2048;
 
 }  // namespace qk
 

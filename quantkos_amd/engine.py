@@ -132,6 +132,14 @@ class Engine:
         self._check(self.L.qk_price_payoffs(self.h, C.byref(c), arr, len(payoffs), out))
         return list(out)
 
+    def price_payoffs_noalloc(self, c: Contract, payoffs) -> list:
+        """price_payoffs through the allocation-free entry point (fixed
+        per-device scratch; QK_ENOMEM beyond it)."""
+        arr = (Payoff * len(payoffs))(*payoffs)
+        out = (Stats * len(payoffs))()
+        self._check(self.L.qk_price_payoffs_noalloc(self.h, C.byref(c), arr, len(payoffs), out))
+        return list(out)
+
     def price_lanes(self, lanes, payoffs) -> list:
         la = (Contract * len(lanes))(*lanes)
         pa = (Payoff * len(payoffs))(*payoffs)
