@@ -39,13 +39,6 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_paths_qe_fp32_4Mx365_hbm_traffi
 
 # VALU issue utilisation of the path kernels from the SQ counter passes.
 VALU_FILE = os.path.join(ROOT, "profiles", "k_paths_qe_4Mx365_valu_counters.md")
-# Semi-analytic Heston price of the European call of the Euler leg (S0=K=1,
-# T=1, r=0.03, v0=0.04, kappa=2, theta=0.04, sigma=0.5, rho=-0.7; Albrecher
-# characteristic function, one 64-point Gauss-Legendre panel on [0, 100]),
-# times the price scale 100.
-HESTON_CF_EUROPEAN = 8.929410423224049
-
-
 def _profile_value(path, key, conv):
     try:
         with open(path) as f:
@@ -164,10 +157,10 @@ def other_configs(engine):
         st = engine.price(c, p)
         wall = time.perf_counter() - t0
         tm = engine.timing()
+        cf = qk.analytic_european(c, qk.CALL, 1.0)      # the product's Heston characteristic-function price
         eu[name] = {"price": st.price, "se": st.se, "wall_ms": wall * 1e3, "kernel_ms": tm.path_kernel_ms,
-                    "path_steps_per_s": 252e6 / wall,
-                    "err_in_se_vs_heston_cf": (st.price - HESTON_CF_EUROPEAN) / st.se}
-    eu["heston_cf"] = HESTON_CF_EUROPEAN
+                    "path_steps_per_s": 252e6 / wall, "err_in_se_vs_heston_cf": (st.price - cf) / st.se}
+    eu["heston_cf"] = cf
     out["heston_euler_european_1Mx252"] = eu
     # Bates QE American put K=110, 500,000 paths, 50 exercise dates.
     progress("Bates QE American put 500k x 50 (FP32, FP64)")
@@ -193,28 +186,47 @@ def other_configs(engine):
                     "backward_gb_s": bw_bytes / (tm.lsm_backward_ms * 1e-3) / 1e9,
                     "backward_us_per_date": tm.lsm_backward_ms * 1e3 / N}
     am["fp32_minus_fp64_in_se"] = (am["fp32"]["price"] - am["fp64"]["price"]) / am["fp64"]["se"]
+    # the weighted Laguerre basis of degree 3 that the workload names
+    # (exp(-X/2) L_k(X), Longstaff and Schwartz 2001; the reference fixes the
+    # unweighted one at compile time, LSMAmerican.hpp:217-233)
+    for prec, name in ((qk.FP32, "fp32"), (qk.FP64, "fp64")):
+        c = qk.contract(model=qk.BATES, scheme=qk.QE, precision=prec, n_steps=50, n_paths=500_000, seed=42,
+                        S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7,
+                        lambda_j=0.1, mu_j=-0.05, sigma_j=0.10, price_scale=100.0)
+        engine.price_american(c, qk.PUT, 1.1, weighted=True)
+        t0 = time.perf_counter()
+        st = engine.price_american(c, qk.PUT, 1.1, weighted=True)
+        wall = time.perf_counter() - t0
+        tm = engine.timing()
+        am["weighted_p03_" + name] = {"price": st.price, "se": st.se, "wall_ms": wall * 1e3,
+                                      "forward_ms": tm.path_kernel_ms, "backward_ms": tm.lsm_backward_ms,
+                                      "backward_us_per_date": tm.lsm_backward_ms * 1e3 / c.n_steps}
     out["bates_qe_american_put_500kx50"] = am
     # Greeks: Heston implicit Milstein, European call K=100 and cash-or-nothing
     # call K=105 from one path set, 2,000,000 paths x 252 steps, S0 +-1 %,
     # vol +-1e-3, r +-1bp, T -+1/365 (central), all bumps as lanes of one launch.
-    progress("Greeks 2M x 252, 13 lanes x 2 contracts (FP32)")
-    c = qk.contract(model=qk.HESTON, scheme=qk.MILSTEIN, precision=qk.FP32, n_steps=252, n_paths=2_000_000,
-                    seed=42, S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7,
-                    price_scale=100.0)
+    progress("Greeks 2M x 252, 13 lanes x 2 contracts (FP32, FP64)")
     pays = [qk.payoff(qk.EUROPEAN, qk.CALL, 1.0), qk.payoff(qk.BINARY_CASH, qk.CALL, 1.05)]
     b = qk.bumps(rel_S=0.01, abs_vol=1e-3, abs_r=1e-4, abs_T=1.0 / 365.0, theta_central=1)
-    engine.greeks(c, pays, b)
-    t0 = time.perf_counter()
-    g = engine.greeks(c, pays, b)
-    wall = time.perf_counter() - t0
-    tm = engine.timing()
-    lanes = g[0].n_lanes
     names = ("delta", "gamma", "vega", "vomma", "vanna", "rho", "theta", "base_price", "base_se")
-    out["heston_milstein_greeks_2Mx252"] = {
-        "lanes": lanes, "contracts": len(pays), "wall_ms": wall * 1e3, "kernel_ms": tm.path_kernel_ms,
-        "path_steps_per_s": lanes * c.n_paths * c.n_steps / wall,
-        "european_call": {k: getattr(g[0], k) for k in names},
-        "binary_cash_call": {k: getattr(g[1], k) for k in names}}
+    gk = {}
+    for prec, name in ((qk.FP32, "fp32"), (qk.FP64, "fp64")):
+        c = qk.contract(model=qk.HESTON, scheme=qk.MILSTEIN, precision=prec, n_steps=252, n_paths=2_000_000,
+                        seed=42, S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5,
+                        rho=-0.7, price_scale=100.0)
+        engine.greeks(c, pays, b)
+        t0 = time.perf_counter()
+        g = engine.greeks(c, pays, b)
+        wall = time.perf_counter() - t0
+        tm = engine.timing()
+        lanes = g[0].n_lanes
+        gk[name] = {"lanes": lanes, "contracts": len(pays), "wall_ms": wall * 1e3, "kernel_ms": tm.path_kernel_ms,
+                    "path_steps_per_s": lanes * c.n_paths * c.n_steps / wall,
+                    "european_call": {k: getattr(g[0], k) for k in names},
+                    "binary_cash_call": {k: getattr(g[1], k) for k in names}}
+    gk["fp32_minus_fp64"] = {ct: {k: gk["fp32"][ct][k] - gk["fp64"][ct][k] for k in names[:-1]}
+                             for ct in ("european_call", "binary_cash_call")}
+    out["heston_milstein_greeks_2Mx252"] = gk
     # The reference's one published benchmark (README.md:324-345,
     # bench_asian_option_price_no_greeks.yaml): Heston QE arithmetic Asian call,
     # 5,000,000 paths x 365 steps, published 5.110195 (SE 0.003025) and
@@ -315,6 +327,13 @@ def serving(engine, n_requests=4096, fp64_fraction=0.25):
     # reported SE (cash flows under a fixed policy) does not measure.
     out["fp32_err_in_se_vs_fp64"] = summary(err)
     out["fp32_err_in_se_vs_fp64_american"] = summary(err_am)
+    # Europeans against the semi-analytic price of their model (Heston or
+    # Bates characteristic function): QE discretisation bias plus noise
+    cf = []
+    for i, (c, p) in enumerate(zip(cs, ps)):
+        if p.type == qk.EUROPEAN and burst[i].stats.se > 0:
+            cf.append((abs(burst[i].stats.price - qk.analytic_european(c, p.right, p.K)) / burst[i].stats.se, i))
+    out["fp32_european_err_in_se_vs_cf"] = summary(cf)
     return out
 
 

@@ -211,6 +211,37 @@ qk_status qk_price_distribution(qk_ctx *ctx, const qk_contract *c, const qk_payo
  * draws the same random numbers whatever path0 and n_paths are. */
 qk_status qk_simulate_paths(qk_ctx *ctx, const qk_contract *c, int64_t path0, double *S_out);
 
+/* qk_price that also keeps its paths: the path kernel writes S after every
+ * step into the ctx's path store [step][path] in the run's precision, as the
+ * reference's StorePaths kernel does when an output file is named
+ * (MSolver.hpp:117-119, Config.hpp:622, :750-758).  The price equals
+ * qk_price's to rounding (the stored form multiplies S step by step). */
+qk_status qk_price_store_paths(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoff, qk_stats *out);
+
+/* Copies paths [path0, path0 + n) of the path set the last qk_price_store_paths,
+ * qk_price_american* or qk_simulate_paths call left on the ctx (path 0 = its
+ * first path) into host memory laid out [path][step] -- the ".paths" payload
+ * (IOBinary.hpp:139-177) -- with elements of the run's precision (float for
+ * QK_FP32, double for QK_FP64): n * n_steps elements.  The transpose runs on
+ * the device (BinWriter's host loop, IOBinary.hpp:155-170). */
+qk_status qk_copy_stored_paths(qk_ctx *ctx, int64_t path0, int64_t n, void *out);
+
+/* ---- semi-analytic European prices (host only; no device work) ----
+ * The reference's closed-form oracles (include/core/analytical:
+ * HestonExact.hpp:174-195, BatesExact.hpp:130-145, BlackScholesExact.hpp:46-65)
+ * as a product call: the European call or put of the contract's model
+ * (Heston, or Bates when model is QK_MODEL_BATES) at strike K, on the same
+ * normalised inputs as the Monte Carlo calls, times price_scale.  Heston and
+ * Bates by characteristic-function inversion (Albrecher et al.'s form),
+ * 64-point Gauss-Legendre panels; quad NULL: panels of 50 until the
+ * integrand has decayed (at most phi = 8000); {100, 100} is the reference's
+ * Heston rule, {8000, 50} its Bates rule.  sigma = 0: Black-Scholes at the
+ * integrated variance (v0 T when kappa = 0), with Merton's jump series for
+ * Bates. */
+typedef struct { double phi_max, panel; } qk_cf_quadrature;
+qk_status qk_analytic_european(const qk_contract *c, int32_t right, double K, const qk_cf_quadrature *quad,
+                               double *price);
+
 /* ---- continuous-batching request server (a stream of pricing requests) ----
  * The reference prices one configuration per process run (app/main.cpp:51-110);
  * these entry points replace a loop of MCDispatcher::launch_montecarlo calls
@@ -246,6 +277,10 @@ qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, qk_server *
 qk_status qk_server_destroy(qk_server *srv);
 /* payoff->type QK_AMERICAN routes the request through Longstaff-Schwartz. */
 qk_status qk_server_submit(qk_server *srv, const qk_contract *c, const qk_payoff *payoff, uint64_t *ticket);
+/* qk_server_submit with the regression basis of an American request (lsm NULL:
+ * LaguerreP03 unweighted, the reference's basis; ignored for other payoffs). */
+qk_status qk_server_submit_ex(qk_server *srv, const qk_contract *c, const qk_payoff *payoff,
+                              const qk_lsm_options *lsm, uint64_t *ticket);
 qk_status qk_server_poll(qk_server *srv, uint64_t ticket, qk_result *out);    /* QK_ENOTREADY if pending */
 qk_status qk_server_wait(qk_server *srv, uint64_t ticket, qk_result *out);
 qk_status qk_server_stats_get(qk_server *srv, qk_server_stats *out);
@@ -276,6 +311,8 @@ typedef struct {
     qk_greeks greeks;          /* zero unless Greeks were requested                 */
     double t_total_s, t_pricing_s, t_greeks_s;   /* the reference's MCTimer lines  */
     qk_distribution dist;      /* MCOpPrices p0 ... p100 (-999 unless MC.analyze_risk_neutral_payoff_distribution) */
+    double analytic_price;     /* European: qk_analytic_european of the same contract; NaN otherwise */
+    double analytic_err_se;    /* (price - analytic_price) / se; NaN when not defined */
 } qk_run_result;
 
 /* MCDispatcher(Parser::parse(file)).launch_montecarlo() (app/main.cpp:78-93,

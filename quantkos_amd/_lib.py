@@ -78,7 +78,12 @@ class RunSpec(C.Structure):
 
 class RunResult(C.Structure):
     _fields_ = [("stats", Stats), ("greeks", Greeks), ("t_total_s", C.c_double),
-                ("t_pricing_s", C.c_double), ("t_greeks_s", C.c_double), ("dist", Distribution)]
+                ("t_pricing_s", C.c_double), ("t_greeks_s", C.c_double), ("dist", Distribution),
+                ("analytic_price", C.c_double), ("analytic_err_se", C.c_double)]
+
+
+class CfQuadrature(C.Structure):
+    _fields_ = [("phi_max", C.c_double), ("panel", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -136,16 +141,20 @@ def load():
         "qk_price_greeks": ([vp, P(Contract), P(Payoff), C.c_int32, P(Bumps), P(Greeks)], st),
         "qk_price_american": ([vp, P(Contract), C.c_int32, C.c_double, P(Stats), P(C.c_double)], st),
         "qk_simulate_paths": ([vp, P(Contract), C.c_int64, P(C.c_double)], st),
+        "qk_price_store_paths": ([vp, P(Contract), P(Payoff), P(Stats)], st),
+        "qk_copy_stored_paths": ([vp, C.c_int64, C.c_int64, vp], st),
         "qk_price_distribution": ([vp, P(Contract), P(Payoff), P(Stats), P(Distribution)], st),
         "qk_price_american_ex": ([vp, P(Contract), C.c_int32, C.c_double, P(LsmOptions), P(Stats),
                                   P(C.c_double)], st),
         "qk_server_create": ([vp, P(ServerConfig), P(vp)], st),
         "qk_server_destroy": ([vp], st),
         "qk_server_submit": ([vp, P(Contract), P(Payoff), P(C.c_uint64)], st),
+        "qk_server_submit_ex": ([vp, P(Contract), P(Payoff), P(LsmOptions), P(C.c_uint64)], st),
         "qk_server_poll": ([vp, C.c_uint64, P(Result)], st),
         "qk_server_wait": ([vp, C.c_uint64, P(Result)], st),
         "qk_server_stats_get": ([vp, P(ServerStats)], st),
         "qk_server_replay": ([vp, P(Contract), P(Payoff), P(C.c_double), C.c_int32, P(Result)], st),
+        "qk_analytic_european": ([P(Contract), C.c_int32, C.c_double, P(CfQuadrature), P(C.c_double)], st),
         "qk_config_load": ([C.c_char_p, P(RunSpec), C.c_char_p, C.c_size_t], st),
         "qk_run_config": ([C.c_char_p, C.c_int, C.c_int32, P(RunResult), C.c_char_p, C.c_size_t], st),
     }

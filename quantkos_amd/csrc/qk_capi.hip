@@ -303,7 +303,7 @@ extern "C" qk_status qk_ctx_create(int device, size_t mem_budget_bytes, qk_ctx *
 qk_ctx::~qk_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (DevBuf *b : {&lanes, &pays, &psum, &pm2, &pitm, &disc, &scale, &stats, &greeks, &store, &cf, &coeffs,
-                      &bpart, &counter, &sums, &mscratch, &payv, &hist})
+                      &bpart, &counter, &sums, &mscratch, &payv, &hist, &tpose})
         b->release();
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
@@ -409,7 +409,7 @@ struct LaneBufs {
 // once the constants block is on the device (or its copy is queued before).
 template <typename Real>
 static qk_status launch_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
-                              const ConstLayout &lay, const LaneBufs &B, Real *pay_out) {
+                              const ConstLayout &lay, const LaneBufs &B, Real *pay_out, Real *store = nullptr) {
     const qk_contract &c0 = lanes[0];
     const bool pathdep = std::any_of(pays, pays + n_pay, [](const qk_payoff &p) { return path_dependent(p.type); });
     const int L = qk_lanes_per_thread(ctx, c0.precision, c0.scheme, n_lanes);
@@ -430,13 +430,19 @@ static qk_status launch_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes
     a.part_sum = B.psum;
     a.part_m2 = B.pm2;
     a.part_itm = B.pitm;
-    a.store = nullptr;
+    a.store = store;
     a.pay_out = pay_out;
 
     QK_HIP(hipEventRecord(ctx->ev[0], s));
+    if (store != nullptr) {
+        // One lane: the path kernel also writes S after every step to the
+        // [step][path] store (the path-dependent form, S multiplied step by
+        // step, whatever the payoff).
+        QK_HIP(launch_paths<Real>(a, c0.model, c0.scheme, true, true, 1, s));
+    }
     // Whole groups of L lanes per thread; the lanes left over go to a launch
     // with L/2 (then 1) lanes per thread instead of being padded.
-    for (int done = 0, Lk = L; done < n_lanes;) {
+    for (int done = store != nullptr ? n_lanes : 0, Lk = L; done < n_lanes;) {
         while (Lk > 1 && n_lanes - done < Lk) Lk >>= 1;
         const int cnt = (n_lanes - done) / Lk * Lk;
         PathArgs<Real> ak = a;
@@ -463,7 +469,7 @@ static qk_status launch_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes
 // ctx->stats on the device.
 template <typename Real>
 static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
-                           Real *pay_out = nullptr) {
+                           Real *pay_out = nullptr, Real *store = nullptr) {
     const ConstLayout lay = const_layout<Real>(n_lanes, n_pay);
     const int64_t n_tiles = (lanes[0].n_paths + TILE - 1) / TILE;
     const size_t groups = (size_t)n_lanes * n_pay;
@@ -482,7 +488,7 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     QK_HIP(hipMemcpyAsync(ctx->lanes.p, ctx->hstage, lay.bytes, hipMemcpyHostToDevice, ctx->stream));
     const LaneBufs B{(char *)ctx->lanes.p, (double *)ctx->psum.p, (double *)ctx->pm2.p, (uint32_t *)ctx->pitm.p,
                      ctx->mscratch.p, (qk_stats *)ctx->stats.p};
-    return launch_lanes<Real>(ctx, lanes, n_lanes, pays, n_pay, lay, B, pay_out);
+    return launch_lanes<Real>(ctx, lanes, n_lanes, pays, n_pay, lay, B, pay_out, store);
 }
 
 static qk_status finish_timing(qk_ctx *ctx, int64_t path_steps, bool has_lsm) {
@@ -570,6 +576,59 @@ extern "C" qk_status qk_price_payoffs(qk_ctx *ctx, const qk_contract *c, const q
 
 extern "C" qk_status qk_price(qk_ctx *ctx, const qk_contract *c, const qk_payoff *p, qk_stats *out) {
     return qk_price_lanes(ctx, c, 1, p, 1, out);
+}
+
+// ---------------------------------------------------------------------------
+// Pricing that keeps its paths (the reference's StorePaths kernel,
+// MSolver.hpp:117-119, taken when an output file is named, Config.hpp:622,
+// :750-758), and the copy of stored paths out in the ".paths" payload layout.
+// ---------------------------------------------------------------------------
+extern "C" qk_status qk_price_store_paths(qk_ctx *ctx, const qk_contract *c, const qk_payoff *p, qk_stats *out) {
+    if (!ctx || !out) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_lanes(ctx, c, 1, p, 1);
+    if (st != QK_OK) return st;
+    if ((st = check_budget(ctx, footprint(*c, 1, 1, true))) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    const size_t rs = c->precision == QK_FP32 ? 4 : 8;
+    ctx->store_paths = 0;
+    if ((st = ctx->store.reserve(ctx, (size_t)c->n_paths * c->n_steps * rs)) != QK_OK) return st;
+    st = c->precision == QK_FP32 ? run_lanes<float>(ctx, c, 1, p, 1, nullptr, (float *)ctx->store.p)
+                                 : run_lanes<double>(ctx, c, 1, p, 1, nullptr, (double *)ctx->store.p);
+    if (st != QK_OK) return st;
+    ctx->store_paths = c->n_paths;
+    ctx->store_steps = c->n_steps;
+    ctx->store_prec = c->precision;
+    QK_HIP(hipMemcpyAsync(ctx->hstage, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, ctx->stream));
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out, ctx->hstage, sizeof(qk_stats));
+    return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, false);
+}
+
+extern "C" qk_status qk_copy_stored_paths(qk_ctx *ctx, int64_t path0, int64_t n, void *out) {
+    if (!ctx || !out) return fail(ctx, QK_EINVAL, "null argument");
+    if (ctx->store_paths <= 0) return fail(ctx, QK_EINVAL, "no stored path set on this ctx");
+    if (path0 < 0 || n < 1 || path0 > ctx->store_paths - n)
+        return fail(ctx, QK_EINVAL, "path window outside the stored path set");
+    const size_t rs = ctx->store_prec == QK_FP32 ? 4 : 8;
+    const int32_t T = ctx->store_steps;
+    // Windows of at most 64 MiB through the transpose buffer.
+    const int64_t win = std::max<int64_t>(1, (int64_t)(64u << 20) / ((int64_t)T * (int64_t)rs));
+    qk_status st;
+    QK_HIP(hipSetDevice(ctx->device));
+    if ((st = ctx->tpose.reserve(ctx, (size_t)std::min(win, n) * T * rs)) != QK_OK) return st;
+    for (int64_t d = 0; d < n; d += win) {
+        const int64_t k = std::min(win, n - d);
+        if (rs == 4)
+            QK_HIP(launch_transpose_paths<float>((const float *)ctx->store.p, ctx->store_paths, T, path0 + d, k,
+                                                 (float *)ctx->tpose.p, ctx->stream));
+        else
+            QK_HIP(launch_transpose_paths<double>((const double *)ctx->store.p, ctx->store_paths, T, path0 + d, k,
+                                                  (double *)ctx->tpose.p, ctx->stream));
+        QK_HIP(hipMemcpyAsync((char *)out + (size_t)d * T * rs, ctx->tpose.p, (size_t)k * T * rs,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        QK_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return QK_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -750,6 +809,7 @@ static qk_status generate_store(qk_ctx *ctx, const qk_contract &c, int64_t path0
     const LaneC<Real> hl = make_lane<Real>(c);
     qk_status st;
     if ((st = ctx->lanes.reserve(ctx, sizeof(LaneC<Real>))) != QK_OK) return st;
+    ctx->store_paths = 0;   // the store holds no complete path set until the launch is queued
     if ((st = ctx->store.reserve(ctx, (size_t)c.n_paths * c.n_steps * sizeof(Real))) != QK_OK) return st;
     QK_HIP(hipMemcpyAsync(ctx->lanes.p, &hl, sizeof(hl), hipMemcpyHostToDevice, ctx->stream));
     PathArgs<Real> a{};
@@ -767,6 +827,9 @@ static qk_status generate_store(qk_ctx *ctx, const qk_contract &c, int64_t path0
     QK_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     QK_HIP(launch_paths<Real>(a, c.model, c.scheme, true, true, 1, ctx->stream));
     QK_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    ctx->store_paths = c.n_paths;
+    ctx->store_steps = c.n_steps;
+    ctx->store_prec = c.precision;
     return QK_OK;
 }
 

@@ -44,6 +44,11 @@ struct qk_ctx {
     qk_timing timing{};
     qk::DevBuf lanes, pays, psum, pm2, pitm, disc, scale, stats, greeks, store, cf, coeffs, bpart, counter, sums, mscratch;
     qk::DevBuf payv, hist;   // per-path payoffs and selection histograms of qk_price_distribution
+    qk::DevBuf tpose;        // [path][step] window of the path store (qk_copy_stored_paths)
+    // Shape of the complete path set in `store` (store_paths = 0: none), left
+    // by qk_price_store_paths, qk_price_american* and qk_simulate_paths.
+    int64_t store_paths = 0;
+    int32_t store_steps = 0, store_prec = QK_FP64;
     std::vector<qk_schedule_entry> schedule;   // lanes per thread by (precision, scheme, n_lanes)
     int32_t lsm_mode = QK_LSM_AUTO;             // backward-pass schedule of qk_price_american*
     // Pinned host staging of a lane call's constants (one host-to-device copy)

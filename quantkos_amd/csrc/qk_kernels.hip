@@ -1014,6 +1014,49 @@ template hipError_t launch_select_hist<float>(const float *, int64_t, const Sele
 template hipError_t launch_select_hist<double>(const double *, int64_t, const SelectArgs &, uint32_t *, int,
                                                hipStream_t);
 
+// ---------------------------------------------------------------------------
+// Path-store transpose for the ".paths" payload: the store is [step][path]
+// (coalesced for the path kernel and the backward pass), the file is
+// [path][step].  One 64 x 64 tile per block through LDS (a padded row keeps
+// the column reads free of bank conflicts); both the reads and the writes
+// are 64 consecutive elements per wavefront.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ __launch_bounds__(256) void k_transpose_paths(const Real *__restrict__ store, int64_t n_paths,
+                                                         int32_t n_steps, int64_t p0, int64_t n,
+                                                         Real *__restrict__ out) {
+    __shared__ Real tile[64][65];
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    const int64_t pb = (int64_t)blockIdx.x * 64;     // first path of the tile (relative to p0)
+    const int tb = (int)blockIdx.y * 64;              // first step of the tile
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int t = tb + ly + 4 * k;
+        const int64_t i = pb + lx;
+        if (t < n_steps && i < n) tile[ly + 4 * k][lx] = store[(int64_t)t * n_paths + p0 + i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int64_t i = pb + ly + 4 * k;
+        const int t = tb + lx;
+        if (t < n_steps && i < n) out[i * n_steps + t] = tile[lx][ly + 4 * k];
+    }
+}
+
+template <typename Real>
+hipError_t launch_transpose_paths(const Real *store, int64_t n_paths, int32_t n_steps, int64_t p0, int64_t n,
+                                  Real *out, hipStream_t s) {
+    if (n <= 0 || p0 < 0 || p0 + n > n_paths || n_steps < 1) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((n_steps + 63) / 64));
+    hipLaunchKernelGGL(k_transpose_paths<Real>, grid, dim3(256), 0, s, store, n_paths, n_steps, p0, n, out);
+    return hipGetLastError();
+}
+template hipError_t launch_transpose_paths<float>(const float *, int64_t, int32_t, int64_t, int64_t, float *,
+                                                  hipStream_t);
+template hipError_t launch_transpose_paths<double>(const double *, int64_t, int32_t, int64_t, int64_t, double *,
+                                                   hipStream_t);
+
 int pump_blocks(int64_t n_paths, int group, int max_blocks, int64_t *chunk) {
     const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((n_paths + group - 1) / group, max_blocks));
     const int64_t c = ((n_paths + nb - 1) / nb + 255) / 256 * 256;

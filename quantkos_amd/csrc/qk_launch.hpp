@@ -141,6 +141,12 @@ hipError_t launch_batch(const ReqD<Real> *reqs, const ItemD *items, int32_t n_it
 hipError_t launch_batch_merge(const MergeReqD *mr, int n_reqs, const int32_t *chunk_req, int64_t n_chunks,
                               qk_stats *out, hipStream_t s);
 
+// Paths [p0, p0 + n) of a [step][path] store (leading dimension n_paths) into
+// out[(i - p0) * n_steps + t]; grid.x = ceil(n / 64) <= 2^32 - 1 blocks.
+template <typename Real>
+hipError_t launch_transpose_paths(const Real *store, int64_t n_paths, int32_t n_steps, int64_t p0, int64_t n,
+                                  Real *out, hipStream_t s);
+
 template <typename Real>
 hipError_t launch_paths(const PathArgs<Real> &a, int model, int scheme, bool pathdep, bool store, int L,
                         hipStream_t s);

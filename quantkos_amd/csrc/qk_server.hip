@@ -92,6 +92,8 @@ struct SReq {
     qk_contract c{};
     qk_payoff p{};
     bool american = false;
+    int n_basis = 4;               // American: Laguerre functions (3, 4, 5 = P02, P03, P04)
+    bool weighted = false;         // American: times exp(-X/2)
     clk::time_point t_submit, t_admit, t_done;
     bool admitted = false;
     qk_status status = QK_OK;
@@ -173,7 +175,7 @@ Layout layout_of(const qk_contract &c, bool american, int64_t n_tiles, int64_t n
     if (american) {
         L.store = take((size_t)c.n_paths * c.n_steps * rs);
         L.cf = take((size_t)c.n_paths * rs);
-        L.coeffs = take((size_t)c.n_steps * 4 * 8);
+        L.coeffs = take((size_t)c.n_steps * LSM_MAX_BASIS * 8);
         L.bpart = take((size_t)2 * nb * LSM_STRIDE * 8);
         L.mreq = take(sizeof(MergeReqD));
         L.creq = take(n_chunks * 4);
@@ -376,21 +378,29 @@ void fill_pump_req(PumpReq<Real> &q, const SReq *r) {
     q.itm_eps = (Real)(r->c.precision == QK_FP32 ? 1e-6 : 1e-12);
 }
 
-// One launch per precision advancing every pump request of that precision by
-// one date.
+// One launch per (precision, regression basis) present, advancing every pump
+// request of that kind by one date (the basis is a template parameter of the
+// date kernel).
 template <typename Real>
 hipError_t launch_pump_step(qk_server *srv, bool fp32) {
-    PumpArgs<Real> a{};
-    int blocks = 0;
-    for (SReq *r : srv->pump_active) {
-        if ((r->c.precision == QK_FP32) != fp32) continue;
-        fill_pump_req<Real>(a.r[a.n_req], r);
-        a.block_start[a.n_req] = blocks;
-        blocks += r->nb;
-        ++a.n_req;
+    for (int nbasis = 3; nbasis <= LSM_MAX_BASIS; ++nbasis) {
+        for (int w = 0; w < 2; ++w) {
+            PumpArgs<Real> a{};
+            int blocks = 0;
+            for (SReq *r : srv->pump_active) {
+                if ((r->c.precision == QK_FP32) != fp32 || r->n_basis != nbasis || (int)r->weighted != w) continue;
+                fill_pump_req<Real>(a.r[a.n_req], r);
+                a.block_start[a.n_req] = blocks;
+                blocks += r->nb;
+                ++a.n_req;
+            }
+            if (a.n_req == 0) continue;
+            a.block_start[a.n_req] = blocks;
+            const hipError_t e = launch_pump<Real>(a, nbasis, w != 0, LSM_SHARED_GROUP, srv->pump_stream);
+            if (e != hipSuccess) return e;
+        }
     }
-    a.block_start[a.n_req] = blocks;
-    return launch_pump<Real>(a, 4, false, LSM_SHARED_GROUP, srv->pump_stream);   // LaguerreP03, the reference's basis
+    return hipSuccess;
 }
 
 // Enqueues the merge of a request whose date-0 step was just queued.
@@ -683,7 +693,14 @@ extern "C" qk_status qk_server_destroy(qk_server *s) {
 }
 
 extern "C" qk_status qk_server_submit(qk_server *s, const qk_contract *c, const qk_payoff *p, uint64_t *ticket) {
+    return qk_server_submit_ex(s, c, p, nullptr, ticket);
+}
+
+extern "C" qk_status qk_server_submit_ex(qk_server *s, const qk_contract *c, const qk_payoff *p,
+                                         const qk_lsm_options *lsm, uint64_t *ticket) {
     if (!s || !c || !p || !ticket) return QK_EINVAL;
+    if (lsm && (lsm->basis < QK_LAGUERRE_P02 || lsm->basis > QK_LAGUERRE_P04))
+        return fail(s->ctx, QK_EINVAL, "lsm basis must be LaguerreP02, P03 or P04");
     qk_ctx *ctx = s->ctx;
     std::lock_guard<std::mutex> lk(s->mu);
     qk_status st = validate_contract(ctx, c);
@@ -700,6 +717,10 @@ extern "C" qk_status qk_server_submit(qk_server *s, const qk_contract *c, const 
     r->c = *c;
     r->p = *p;
     r->american = american;
+    if (lsm) {
+        r->n_basis = 3 + lsm->basis;
+        r->weighted = lsm->weighted != 0;
+    }
     r->n_tiles = (c->n_paths + TILE - 1) / TILE;
     r->n_chunks = (r->n_tiles + MERGE_CHUNK - 1) / MERGE_CHUNK;
     if (american)

@@ -48,6 +48,21 @@ def bumps(**kw) -> Bumps:
     return b
 
 
+def analytic_european(c: Contract, right: int, K: float, quadrature=None) -> float:
+    """Semi-analytic European price of the contract's model (host only; Heston /
+    Bates characteristic function, Black-Scholes / Merton when sigma = 0), in
+    the contract's price scale.  quadrature: None (panels of 50 until the
+    integrand decays) or (phi_max, panel), e.g. (100, 100) for the
+    reference's Heston rule."""
+    L = _lib.load()
+    out = C.c_double()
+    q = None if quadrature is None else C.byref(_lib.CfQuadrature(*quadrature))
+    st = L.qk_analytic_european(C.byref(c), right, K, q, C.byref(out))
+    if st != 0:
+        raise QKError(st, "qk_analytic_european")
+    return out.value
+
+
 def load_config(path: str):
     """Config::Parser::parse + apply_price_scaling of a QuantKos YAML file (no device work)."""
     L = _lib.load()
@@ -183,6 +198,19 @@ class Engine:
                                              S.ctypes.data_as(C.POINTER(C.c_double))))
         return S
 
+    def price_store_paths(self, c: Contract, p: Payoff) -> Stats:
+        """qk_price that keeps its path set on the device (for stored_paths)."""
+        out = Stats()
+        self._check(self.L.qk_price_store_paths(self.h, C.byref(c), C.byref(p), C.byref(out)))
+        return out
+
+    def stored_paths(self, path0: int, n: int, n_steps: int, precision: int) -> np.ndarray:
+        """Paths [path0, path0 + n) of the last stored path set, [path][step],
+        float32 for FP32 runs and float64 for FP64 runs."""
+        a = np.empty((n, n_steps), dtype=np.float32 if precision == _lib.FP32 else np.float64)
+        self._check(self.L.qk_copy_stored_paths(self.h, path0, n, a.ctypes.data_as(C.c_void_p)))
+        return a
+
     def timing(self) -> Timing:
         t = Timing()
         self._check(self.L.qk_last_timing(self.h, C.byref(t)))
@@ -219,9 +247,16 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, c: Contract, p: Payoff) -> int:
+    def submit(self, c: Contract, p: Payoff, basis: int = _lib.LAGUERRE_P03, weighted: bool = False) -> int:
+        """Ticket of a request; basis / weighted choose the regression basis of
+        an American request (LaguerreP03 unweighted by default)."""
         t = C.c_uint64()
-        self.engine._check(self.L.qk_server_submit(self.h, C.byref(c), C.byref(p), C.byref(t)))
+        if basis == _lib.LAGUERRE_P03 and not weighted:
+            self.engine._check(self.L.qk_server_submit(self.h, C.byref(c), C.byref(p), C.byref(t)))
+        else:
+            opt = _lib.LsmOptions(basis, int(bool(weighted)))
+            self.engine._check(self.L.qk_server_submit_ex(self.h, C.byref(c), C.byref(p), C.byref(opt),
+                                                          C.byref(t)))
         return t.value
 
     def poll(self, ticket: int):

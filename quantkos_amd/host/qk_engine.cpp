@@ -157,6 +157,9 @@ void OptionPricer::evaluate_option_price() {
         metrics.p99 = dist.p99;
         metrics.p100 = dist.p100;
     }
+    metrics.analytic_price = analytic;
+    metrics.analytic_err_se = std::isfinite(analytic) && stats.se > 0.0 ? (stats.price - analytic) / stats.se
+                                                                         : std::numeric_limits<double>::quiet_NaN();
     metrics.is_computed = true;
 }
 
@@ -194,6 +197,13 @@ void OptionPricer::print_info_option_price() const {
                   << "  --------------------------------------------------------\n"
                   << "   Time Spent to Sorting the Payoffs   :  " << dist.select_s << "s \n";
     }
+    if (std::isfinite(metrics.analytic_price)) {
+        std::cout << "  --------------------------------------------------------\n"
+                  << "   Semi-analytic Price (CF) :  " << metrics.analytic_price << "\n"
+                  << "   MC - Semi-analytic       :  " << metrics.option_price - metrics.analytic_price << "  ("
+                  << std::setprecision(2) << metrics.analytic_err_se << " SE)\n"
+                  << std::setprecision(6);
+    }
     std::cout << "  ========================================================\n" << std::endl;
     std::cout.copyfmt(old);
 }
@@ -206,18 +216,28 @@ const OptionPricer::MCOpPrices &OptionPricer::get_option_price_data() const {
 // --------------------------------------------------------------------------
 // Runners.
 // --------------------------------------------------------------------------
-OptionPricer::MCOpPrices price_forward(const KC::UInputs &scaled, bool print) {
+OptionPricer::MCOpPrices price_forward(const KC::UInputs &scaled, bool print, bool *stored) {
     qk_ctx *ctx = device_context();
     const qk_contract c = to_contract(scaled);
     const qk_payoff p = to_payoff(scaled);
     qk_stats s{};
     OptionPricer pr(scaled);
+    const bool keep_paths = !scaled.output.filename_paths_out.empty();
+    if (stored) *stored = false;
     if (scaled.mc.analyze_risk_neutral_payoff_distribution) {
         qk_distribution d{};
         check(qk_price_distribution(ctx, &c, &p, &s, &d), "qk_price_distribution");
         pr.accumulate_device_distribution(d);
+    } else if (keep_paths) {
+        // the reference's StorePaths kernel when an output file is named (MSolver.hpp:117-119)
+        check(qk_price_store_paths(ctx, &c, &p, &s), "qk_price_store_paths");
+        if (stored) *stored = true;
     } else {
         check(qk_price(ctx, &c, &p, &s), "qk_price");
+    }
+    if (p.type == QK_EUROPEAN) {
+        double a = 0.0;
+        if (qk_analytic_european(&c, p.right, p.K, nullptr, &a) == QK_OK) pr.set_analytic_price(a);
     }
     pr.accumulate_device_stats(s);
     pr.evaluate_option_price();
@@ -253,8 +273,9 @@ MCResults run_forward(KC::UInputs config) {
     MCTimer timer;
     timer.start_total();
     timer.start_pricing();
-    const auto base = price_forward(config, true);
-    if (!config.output.filename_paths_out.empty()) write_paths_file(config);
+    bool stored = false;
+    const auto base = price_forward(config, true, &stored);
+    if (!config.output.filename_paths_out.empty()) write_paths_file(config, stored);
     timer.stop_pricing();
     MCGreeks g{};
     int32_t lanes = 0;
@@ -283,7 +304,11 @@ MCResults run_forward_backward(KC::UInputs config) {
     MCResults r = execute_mc_framework(config, [&](const KC::UInputs &c) {
         const bool print = first;
         first = false;
-        return price_american(c, print);
+        auto m = price_american(c, print);
+        // the base pricing's path store is the path set of the output file
+        // (the Greek repricings name none, MCEngine.hpp:583)
+        if (print && !c.output.filename_paths_out.empty()) write_paths_file(c, true);
+        return m;
     });
     if (config.must_compute_greeks()) r.Greeks.print_summary();
     r.Timing.print_summary();

@@ -13,6 +13,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <string>
 
 #include "qk.h"
@@ -61,6 +62,11 @@ public:
                median = -999., p60 = -999., p70 = -999., p80 = -999., p90 = -999., p95 = -999., p99 = -999.,
                p100 = -999.;
         bool is_computed = false;
+        // European options: the semi-analytic price of the model (Heston / Bates
+        // characteristic function) and the Monte Carlo price's distance from it
+        // in standard errors; NaN for other payoffs.
+        double analytic_price = std::numeric_limits<double>::quiet_NaN();
+        double analytic_err_se = std::numeric_limits<double>::quiet_NaN();
     };
     explicit OptionPricer(const KC::UInputs &config) : config(config) {}
     // Takes the device's merged statistics of the whole path set (the
@@ -68,6 +74,7 @@ public:
     void accumulate_device_stats(const qk_stats &s) { stats = s; }
     // The device-selected distribution of the same paths (analyze_risk_neutral_
     // discounted_payoff_distribution, OptionPricer.hpp:340-360).
+    void set_analytic_price(double p) { analytic = p; }
     void accumulate_device_distribution(const qk_distribution &d) {
         dist = d;
         has_dist = true;
@@ -81,6 +88,7 @@ private:
     qk_stats stats{};
     qk_distribution dist{};
     bool has_dist = false;
+    double analytic = std::numeric_limits<double>::quiet_NaN();
     MCOpPrices metrics;
 };
 using MCOpPrices = OptionPricer::MCOpPrices;
@@ -101,7 +109,9 @@ template <typename PricingCallable>
 MCResults execute_mc_framework(const KC::UInputs &config, PricingCallable &&pricing_function);
 
 // One pricing of a scaled configuration through the device.
-OptionPricer::MCOpPrices price_forward(const KC::UInputs &scaled, bool print);
+// stored (if not null) is set when the call left the path set in the ctx's
+// path store (an output file is named and no distribution was analysed).
+OptionPricer::MCOpPrices price_forward(const KC::UInputs &scaled, bool print, bool *stored = nullptr);
 OptionPricer::MCOpPrices price_american(const KC::UInputs &scaled, bool print);
 MCResults run_forward(KC::UInputs config);
 MCResults run_forward_backward(KC::UInputs config);

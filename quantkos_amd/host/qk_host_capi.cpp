@@ -92,6 +92,8 @@ extern "C" qk_status qk_run_config(const char *yaml_path, int device, int32_t qu
         out->t_total_s = r.Timing.s_elapsed_total;
         out->t_pricing_s = r.Timing.s_elapsed_for_pricing;
         out->t_greeks_s = r.Timing.s_elapsed_for_greeks;
+        out->analytic_price = r.OptionPrice.analytic_price;
+        out->analytic_err_se = r.OptionPrice.analytic_err_se;
         return QK_OK;
     } catch (const std::exception &e) {
         copy_err(err, err_len, e.what());

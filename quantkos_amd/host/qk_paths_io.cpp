@@ -29,39 +29,34 @@ void put_header(char *h, int precision_bytes, int n_sims, int n_steps, double dt
 }
 }  // namespace
 
-std::string write_paths_file(const Config::UInputs &cfg) {
+std::string write_paths_file(const Config::UInputs &cfg, bool stored) {
     const std::filesystem::path file = std::filesystem::path(cfg.output.out_dir) / cfg.output.filename_paths_out;
     std::ofstream out(file, std::ios::out | std::ios::binary);
     if (!out) throw std::runtime_error("[BinWriter Error] Failed to open binary output file: " + file.string());
-    qk_contract c = to_contract(cfg);
+    const qk_contract c = to_contract(cfg);
+    qk_ctx *ctx = device_context();
+    if (!stored) {
+        const qk_payoff p = to_payoff(cfg);
+        qk_stats s{};
+        if (qk_price_store_paths(ctx, &c, &p, &s) != QK_OK)
+            throw std::runtime_error(std::string("qk_price_store_paths failed: ") + qk_last_error(ctx));
+    }
     const bool fp32 = c.precision == QK_FP32;
+    const size_t rs = fp32 ? 4 : 8;
     char header[HEADER_BYTES];
-    put_header(header, fp32 ? 4 : 8, cfg.mc.N_Paths, cfg.time.N_time_steps, cfg.time.dt, cfg.get_price_scaling_factor());
+    put_header(header, (int)rs, cfg.mc.N_Paths, cfg.time.N_time_steps, cfg.time.dt, cfg.get_price_scaling_factor());
     out.write(header, HEADER_BYTES);
-    // Path ranges of at most ~64 MiB of host doubles each.
+    // The device hands back row-major [path][step] windows of ~64 MiB in the
+    // run's precision: the payload is written as it arrives.
     const int64_t N = cfg.mc.N_Paths, T = cfg.time.N_time_steps;
-    const int64_t chunk = std::max<int64_t>(1, (int64_t)(8u << 20) / std::max<int64_t>(T, 1));
-    std::vector<double> buf;
-    std::vector<char> row((size_t)T * (fp32 ? 4 : 8));
+    const int64_t chunk = std::max<int64_t>(1, (int64_t)(64u << 20) / (T * (int64_t)rs));
+    std::vector<char> buf;
     for (int64_t p0 = 0; p0 < N; p0 += chunk) {
         const int64_t n = std::min(chunk, N - p0);
-        c.n_paths = n;
-        buf.resize((size_t)(n * T));
-        const qk_status st = qk_simulate_paths(device_context(), &c, p0, buf.data());
-        if (st != QK_OK)
-            throw std::runtime_error(std::string("qk_simulate_paths failed: ") + qk_last_error(device_context()));
-        for (int64_t i = 0; i < n; ++i) {   // [step][path] -> row-major [path][step]
-            for (int64_t t = 0; t < T; ++t) {
-                const double v = buf[(size_t)(t * n + i)];
-                if (fp32) {
-                    const float f = (float)v;
-                    std::memcpy(row.data() + t * 4, &f, 4);
-                } else {
-                    std::memcpy(row.data() + t * 8, &v, 8);
-                }
-            }
-            out.write(row.data(), (std::streamsize)row.size());
-        }
+        buf.resize((size_t)(n * T) * rs);
+        if (qk_copy_stored_paths(ctx, p0, n, buf.data()) != QK_OK)
+            throw std::runtime_error(std::string("qk_copy_stored_paths failed: ") + qk_last_error(ctx));
+        out.write(buf.data(), (std::streamsize)buf.size());
     }
     if (!out) throw std::runtime_error("[BinWriter Error] write failed: " + file.string());
     return file.string();

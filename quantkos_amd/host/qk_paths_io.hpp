@@ -16,10 +16,12 @@
 
 namespace quantkos::Engine {
 
-// Simulates the configuration's path set on the device and writes
-// out_dir/Name_Paths_Out_File.  `scaled` is the configuration after
+// Writes out_dir/Name_Paths_Out_File from the path set the pricing call left
+// in the device ctx's path store (stored = true: qk_price_store_paths or
+// qk_price_american* just ran on `scaled`), or prices once more with the
+// store when stored = false.  `scaled` is the configuration after
 // apply_price_scaling.  Returns the file path.
-std::string write_paths_file(const Config::UInputs &scaled);
+std::string write_paths_file(const Config::UInputs &scaled, bool stored);
 
 struct PathsFile {
     int version = 0, byte_precision = 0, total_n_sims = 0, n_time_steps = 0;

@@ -128,3 +128,62 @@ def test_yaml_payoff_distribution(in_tmp):
             d.p100]
     assert vals[0] == 0.0 and all(a <= b for a, b in zip(vals, vals[1:])) and vals[-1] > r.stats.price
     assert run_config(off).dist.median == -999.0
+
+
+@pytest.mark.parametrize("precision", [qk.FP32, qk.FP64])
+@pytest.mark.parametrize("scheme", [qk.EULER, qk.QE])
+def test_store_paths_pricing(precision, scheme):
+    """qk_price_store_paths keeps the pricing launch's paths (the reference's
+    StorePaths kernel, MSolver.hpp:117-119); qk_copy_stored_paths hands them back
+    [path][step] in the run's precision: equal to qk_simulate_paths' paths of the
+    same set, for the whole set and for a window; the price equals qk_price's."""
+    c = qk.contract(model=qk.BATES, scheme=scheme, precision=precision, n_steps=70, n_paths=5_003, seed=11,
+                    S0=1.0, v0=0.04, r=0.03, q=0.01, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7,
+                    lambda_j=0.5, price_scale=100.0)
+    p = qk.payoff(qk.BARRIER_UP_OUT, qk.CALL, 1.0, 1.3)
+    with qk.Engine(0) as e:
+        s = e.price_store_paths(c, p)
+        every = e.stored_paths(0, c.n_paths, c.n_steps, precision)
+        win = e.stored_paths(1_234, 1_000, c.n_steps, precision)
+        ref = e.price(c, p)
+        S = e.simulate_paths(c)
+    assert every.dtype == (np.float32 if precision == qk.FP32 else np.float64)
+    assert np.array_equal(every.astype(np.float64), S.T)
+    assert np.array_equal(win, every[1_234:2_234])
+    assert s.n == ref.n == c.n_paths and s.itm == ref.itm
+    assert math.isclose(s.price, ref.price, rel_tol=1e-12 if precision == qk.FP64 else 1e-6)
+
+
+def test_american_store_is_the_priced_path_set():
+    c = qk.contract(model=qk.HESTON, scheme=qk.QE, precision=qk.FP32, n_steps=50, n_paths=20_000, seed=4,
+                    S0=1.0, v0=0.04, r=0.03, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7)
+    with qk.Engine(0) as e:
+        e.price_american(c, qk.PUT, 1.1)
+        got = e.stored_paths(0, c.n_paths, c.n_steps, qk.FP32)
+        S = e.simulate_paths(c)
+    assert np.array_equal(got.astype(np.float64), S.T)
+
+
+def test_american_yaml_writes_paths_and_european_reports_cf_price(in_tmp):
+    """The forward-backward runner writes the base pricing's path set; a European
+    run reports the semi-analytic price of its model beside the Monte Carlo one."""
+    text = open(os.path.join(GOLDEN, "bates_barrier_paths.yaml")).read()
+    am = text.replace("OptionType: BarrierUpAndOut", "OptionType: American")
+    assert am != text
+    (in_tmp / "am.yaml").write_text(am)
+    r = run_config(str(in_tmp / "am.yaml"))
+    assert math.isnan(r.analytic_price)
+    f = read_paths(in_tmp / "qk_test_out" / "bates.paths")
+    spec = load_config(str(in_tmp / "am.yaml"))
+    with qk.Engine(0) as e:
+        S = e.simulate_paths(spec.contract)
+    assert np.array_equal(f["data"], S.T)
+    eu = text.replace("OptionType: BarrierUpAndOut", "OptionType: European").replace(
+        "N_Paths: 1000", "N_Paths: 400000")
+    (in_tmp / "eu.yaml").write_text(eu)
+    r = run_config(str(in_tmp / "eu.yaml"))
+    spec = load_config(str(in_tmp / "eu.yaml"))
+    c = spec.contract
+    want = qk.analytic_european(c, spec.payoff.right, spec.payoff.K)
+    assert r.analytic_price == pytest.approx(want, rel=1e-12)
+    assert r.analytic_err_se == pytest.approx((r.stats.price - want) / r.stats.se, rel=1e-9)

@@ -200,3 +200,31 @@ def test_config4_stream_full_size(engine):
             continue
         worst = max(worst, abs(a.stats.price - b.stats.price) / b.stats.se)
     assert worst <= 1.0, worst
+
+
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+def test_server_american_bases_match_plain_launch(engine, precision):
+    """American requests with each regression basis (qk_server_submit_ex) in one
+    mix: the pump advances each (precision, basis) group with its own date
+    kernel, and every result agrees with qk_price_american_ex's for that basis.
+    The server's pump splits the paths into other blocks than the plain call,
+    so the Gram sums differ in rounding; with the worse-conditioned bases that
+    can flip an exercise decision near the continuation value, hence 0.05
+    standard errors there and the plain-launch tolerance for LaguerreP03."""
+    cases = [(qk.LAGUERRE_P02, False), (qk.LAGUERRE_P03, False), (qk.LAGUERRE_P04, False),
+             (qk.LAGUERRE_P03, True), (qk.LAGUERRE_P04, True)]
+    cs = [qk.contract(model=qk.BATES, scheme=qk.QE, precision=precision, n_steps=30 + 7 * i, n_paths=20_000 + 999 * i,
+                      seed=50 + i, S0=1.0, v0=0.04, r=0.05, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7)
+          for i in range(len(cases))]
+    p = qk.payoff(qk.AMERICAN, qk.PUT, 1.1)
+    with Server(engine) as srv:
+        tickets = [srv.submit(c, p, basis=b, weighted=w) for c, (b, w) in zip(cs, cases)]
+        res = [srv.wait(t) for t in tickets]
+    tol = 1e-12 if precision == qk.FP64 else 1e-5
+    for c, (b, w), r in zip(cs, cases, res):
+        assert r.status == 0
+        ref = engine.price_american(c, qk.PUT, 1.1, basis=b, weighted=w)
+        if b == qk.LAGUERRE_P03 and not w:
+            assert math.isclose(r.stats.price, ref.price, rel_tol=tol, abs_tol=1e-12), (r.stats.price, ref.price)
+        else:
+            assert abs(r.stats.price - ref.price) <= 0.05 * ref.se, (b, w, r.stats.price, ref.price, ref.se)
