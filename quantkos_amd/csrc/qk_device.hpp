@@ -631,10 +631,60 @@ __device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, ui
 // ---------------------------------------------------------------------------
 // Wavefront (64-lane) reductions.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Cross-lane exchanges of a 64-lane wavefront without LDS traffic (every lane
+// of the wavefront must be active).  DPP moves inside rows of 16 lanes:
+// QP_XOR1 / QP_XOR2 take lane l ^ 1 / l ^ 2 (quad permutations), HALF_MIRROR
+// lane 7 - l and MIRROR lane 15 - l within each group of 8 / 16.  Across rows,
+// gfx950's permlane swaps exchange whole halves of two registers: after
+// swap32(a, b), a + b holds a's sum over lanes l, l + 32 in lanes 0..31 and
+// b's in lanes 32..63; swap16 does the same for row pairs (0,1) and (2,3).
+// ---------------------------------------------------------------------------
+enum : int { DPP_QP_XOR1 = 0xB1, DPP_QP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140 };
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+template <bool ROWS16>
+__device__ __forceinline__ void swap_halves_f64(double &a, double &b) {
+    const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
+    const unsigned alo = (unsigned)ua, ahi = (unsigned)(ua >> 32), blo = (unsigned)ub, bhi = (unsigned)(ub >> 32);
+    unsigned nlo_a, nlo_b, nhi_a, nhi_b;
+    if constexpr (ROWS16) {
+        const auto lo = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+        nlo_a = lo[0]; nlo_b = lo[1]; nhi_a = hi[0]; nhi_b = hi[1];
+    } else {
+        const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+        nlo_a = lo[0]; nlo_b = lo[1]; nhi_a = hi[0]; nhi_b = hi[1];
+    }
+    a = __longlong_as_double((long long)(((uint64_t)nhi_a << 32) | nlo_a));
+    b = __longlong_as_double((long long)(((uint64_t)nhi_b << 32) | nlo_b));
+}
+
+// x + x of the partner lane l ^ 16 (ROWS16) or l ^ 32; the same bits in both.
+template <bool ROWS16>
+__device__ __forceinline__ double add_across(double x) {
+    double a = x, b = x;
+    swap_halves_f64<ROWS16>(a, b);
+    return a + b;
+}
+
+// Sum over the wavefront, the same bits in every lane (each step adds two
+// lanes' values, which both lanes of the pair compute in the same order).
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
+    x += dpp_f64<DPP_QP_XOR1>(x);
+    x += dpp_f64<DPP_QP_XOR2>(x);
+    x += dpp_f64<DPP_HALF_MIRROR>(x);
+    x += dpp_f64<DPP_MIRROR>(x);
+    x = add_across<true>(x);
+    return add_across<false>(x);
 }
 
 // Chan et al. pairwise merge of (n, mean, M2) moments.

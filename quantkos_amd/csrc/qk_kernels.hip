@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "qk_device.hpp"
 #include "qk_launch.hpp"
@@ -449,8 +450,8 @@ __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpar
     // lanes 0 .. ST-1; one barrier; then lane k of wavefront 0 adds component
     // k over the wavefronts in order and thread 0 gathers the NS sums.
     const int lid = lt & 63, wv = lt >> 6;
-#pragma unroll
-    for (int o = ST; o < 64; o <<= 1) tot += __shfl_xor(tot, o, 64);
+    if constexpr (ST == 16) tot = add_across<true>(tot);
+    tot = add_across<false>(tot);
     if (active && lid < ST) red[wv * ST + lid] = tot;
     __syncthreads();
     if (active && wv == 0) {
@@ -458,8 +459,13 @@ __device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpar
         if (lid < ST)
             for (int w = 0; w < G / 64; ++w) v += red[w * ST + lid];
         double sm[NS];
+        const uint64_t vb = (uint64_t)__double_as_longlong(v);
 #pragma unroll
-        for (int k = 0; k < NS; ++k) sm[k] = __shfl(v, k, 64);
+        for (int k = 0; k < NS; ++k) {   // lane k's component into scalar registers
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)vb, k);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(vb >> 32), k);
+            sm[k] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+        }
         if (lid == 0) lsm_solve<NB>(sm, C);
     }
 }
@@ -601,10 +607,13 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
     }
     if (t >= 1) {
         // Wavefront sums of the NS components by a reduce-scatter butterfly
-        // over KP = 16 or 32 padded components: at lane offsets 32, 16, ...
-        // each lane keeps half of its values and sends the other half, so KP
-        // components cost KP - 1 shuffles (plus 6 - log2 KP full steps)
-        // instead of 6 per component.  Lane l ends with component
+        // over KP = 16 or 32 padded components: at each level every lane
+        // keeps half of its values and adds the partner's copy of them, so
+        // KP components cost KP - 1 exchanges (plus 6 - log2 KP full steps)
+        // instead of 6 per component.  The partners differ in lane bit 5, 4
+        // (permlane swaps, which need no select), 3, 2 (row mirrors) and 1, 0
+        // (quad permutations); those masks span all six bits, so every sum
+        // covers the 64 lanes once.  Lane l ends with component
         // l >> (6 - log2 KP) summed over the wavefront.
         constexpr int KP = NS <= 16 ? 16 : 32;
         constexpr int LOGK = KP == 16 ? 4 : 5;
@@ -612,18 +621,36 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
 #pragma unroll
         for (int k = 0; k < KP; ++k) v[k] = k < NS ? g[k] : 0.0;
 #pragma unroll
-        for (int o = 32, h = KP / 2; h >= 1; o >>= 1, h >>= 1) {
-            const bool hi = (lid & o) != 0;
+        for (int k = 0; k < KP / 2; ++k) {            // bit 5: lanes >= 32 keep k + KP/2
+            swap_halves_f64<false>(v[k], v[k + KP / 2]);
+            v[k] += v[k + KP / 2];
+        }
+#pragma unroll
+        for (int k = 0; k < KP / 4; ++k) {            // bit 4
+            swap_halves_f64<true>(v[k], v[k + KP / 4]);
+            v[k] += v[k + KP / 4];
+        }
+        // bits 3, 2 (and 1 for KP = 32): the lane with the bit set keeps the
+        // upper half and sends the lower one.
+        auto level = [&](auto ctrl, int h, int bit) {
+            const bool up = (lid & bit) != 0;
 #pragma unroll
             for (int k = 0; k < h; ++k) {
-                const double keep = hi ? v[k + h] : v[k];
-                const double send = hi ? v[k] : v[k + h];
-                v[k] = keep + __shfl_xor(send, o, 64);
+                const double keep = up ? v[k + h] : v[k];
+                const double send = up ? v[k] : v[k + h];
+                v[k] = keep + dpp_f64<decltype(ctrl)::value>(send);
             }
+        };
+        level(std::integral_constant<int, DPP_MIRROR>{}, KP / 8, 8);
+        level(std::integral_constant<int, DPP_HALF_MIRROR>{}, KP / 16, 4);
+        double sx;
+        if constexpr (KP == 32) {
+            level(std::integral_constant<int, DPP_QP_XOR2>{}, 1, 2);
+            sx = v[0] + dpp_f64<DPP_QP_XOR1>(v[0]);
+        } else {
+            sx = v[0] + dpp_f64<DPP_QP_XOR2>(v[0]);
+            sx += dpp_f64<DPP_QP_XOR1>(sx);
         }
-        double sx = v[0];
-#pragma unroll
-        for (int o = (64 >> LOGK) / 2; o >= 1; o >>= 1) sx += __shfl_xor(sx, o, 64);
         const int comp = lid >> (6 - LOGK);
         if ((lid & ((1 << (6 - LOGK)) - 1)) == 0 && comp < NS) wpart[wv][comp] = sx;
         __syncthreads();
