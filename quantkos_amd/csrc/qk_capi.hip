@@ -242,9 +242,12 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
         b = sat_add(b, sat_mul(sat_mul((size_t)c.n_paths, (size_t)c.n_steps), rs));   // path store [date][path]
         b = sat_add(b, sat_mul((size_t)c.n_paths, rs));                                // cash flows
         int64_t chunk = 0;
-        // Gram partials: the smaller block size gives the larger block count.
+        // Gram partials: as many as the larger of the plain call's and the
+        // request server's block counts.
         b = sat_add(b, sat_mul((size_t)c.n_steps, LSM_MAX_BASIS * 8));
-        b = sat_add(b, (size_t)2 * pump_blocks(c.n_paths, LSM_SHARED_GROUP, PUMP_MAX_BLOCKS, &chunk) * LSM_STRIDE * 8);
+        const int nb = std::max(pump_blocks(c.n_paths, LSM_SHARED_GROUP, PUMP_MAX_BLOCKS, &chunk),
+                                server_pump_blocks(c.n_paths, &chunk));
+        b = sat_add(b, (size_t)2 * nb * LSM_STRIDE * 8);
     }
     return b;
 }

@@ -703,6 +703,50 @@ __global__ __launch_bounds__(G) void k_lsm_step(PumpArgs<Real> a) {
                                           part, q.psum, q.pm2, q.pitm, wpart, pre);
 }
 
+// The request server's Longstaff-Schwartz step in two launches, for requests
+// split into many blocks: k_lsm_solve has one block per request, which sums
+// the request's nb Gram partials of date t in block order, solves the normal
+// equations and stores the date's coefficients; k_lsm_paths then runs the
+// date over every block's chunk with those coefficients (read after the
+// kernel boundary) and writes its partial of date t-1.  The solve no longer
+// sits in every block, so a request can take thousands of blocks and the
+// path work reaches the bandwidth a fused step with 256 re-read partials per
+// block cannot (the server's pump steps share the GPU with batch launches and
+// are long enough that one more launch per step costs nothing measurable).
+template <typename Real, int NB, int G>
+__global__ __launch_bounds__(G) void k_lsm_solve(PumpArgs<Real> a) {
+    using S = LsmSums<NB>;
+    const PumpReq<Real> &q = a.r[blockIdx.x];
+    const int t = q.t;
+    if (t >= q.n_steps - 1) return;                   // uniform per block: the last date needs no solve
+    __shared__ double red[G];
+    double C[NB];
+    lsm_reduce_solve<NB, G>(q.bpart + (int64_t)(t & 1) * q.nb * S::STRIDE, q.nb, red, true, C);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NB; ++k) q.coeffs[(int64_t)t * NB + k] = C[k];
+}
+
+template <typename Real, int NB, bool W, int G>
+__global__ __launch_bounds__(G) void k_lsm_paths(PumpArgs<Real> a) {
+    using S = LsmSums<NB>;
+    int r = 0;
+    while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
+    const PumpReq<Real> &q = a.r[r];
+    const int b = (int)blockIdx.x - a.block_start[r];
+    const int t = q.t, N = q.n_steps;
+    __shared__ double wpart[G / 64][S::NS];
+    auto pre = [&](double C[NB]) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? q.coeffs[(int64_t)t * NB + k] : 0.0;
+    };
+    double C[NB];
+    const int64_t p0 = (int64_t)b * q.chunk;
+    const int64_t p1 = p0 + q.chunk < q.n_paths ? p0 + q.chunk : q.n_paths;
+    double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * S::STRIDE + (int64_t)b * S::STRIDE : nullptr;
+    lsm_chunk_date<Real, NB, W, false, G>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
+                                          part, q.psum, q.pm2, q.pitm, wpart, pre);
+}
+
 // Synchronisation of the cooperative backward pass (sync words, agent scope):
 // arrivals are counted per group of blocks (block b in group b % G_R, one
 // counter per 64-byte line) and groups on a top counter, so no address takes
@@ -1145,6 +1189,38 @@ template int lsm_coop_occupancy<float>(int, bool, size_t);
 template int lsm_coop_occupancy<double>(int, bool, size_t);
 template hipError_t launch_lsm_coop<float>(const LsmCoopArgs<float> &, int, bool, int, size_t, hipStream_t);
 template hipError_t launch_lsm_coop<double>(const LsmCoopArgs<double> &, int, bool, int, size_t, hipStream_t);
+
+template <typename Real, int NB, bool W>
+static void launch_pump2_nb(const PumpArgs<Real> &a, hipStream_t s) {
+    hipLaunchKernelGGL((k_lsm_solve<Real, NB, LSM_SOLVE_GROUP>), dim3(a.n_req), dim3(LSM_SOLVE_GROUP), 0, s, a);
+    hipLaunchKernelGGL((k_lsm_paths<Real, NB, W, LSM_SHARED_GROUP>), dim3(a.block_start[a.n_req]),
+                       dim3(LSM_SHARED_GROUP), 0, s, a);
+}
+
+template <typename Real>
+hipError_t launch_pump_two_phase(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s) {
+    if (a.n_req <= 0) return hipSuccess;
+    switch (n_basis * 2 + (weighted ? 1 : 0)) {
+    case 6: launch_pump2_nb<Real, 3, false>(a, s); break;
+    case 7: launch_pump2_nb<Real, 3, true>(a, s); break;
+    case 8: launch_pump2_nb<Real, 4, false>(a, s); break;
+    case 9: launch_pump2_nb<Real, 4, true>(a, s); break;
+    case 10: launch_pump2_nb<Real, 5, false>(a, s); break;
+    case 11: launch_pump2_nb<Real, 5, true>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_pump_two_phase<float>(const PumpArgs<float> &, int, bool, hipStream_t);
+template hipError_t launch_pump_two_phase<double>(const PumpArgs<double> &, int, bool, hipStream_t);
+
+int server_pump_blocks(int64_t n_paths, int64_t *chunk) {
+    // one pass of 8 rows of 256 threads per block, at most SERVER_PUMP_MAX_BLOCKS blocks
+    const int64_t rows = (n_paths + SERVER_PUMP_MAX_BLOCKS - 1) / SERVER_PUMP_MAX_BLOCKS;
+    const int64_t c = std::max<int64_t>(SERVER_PUMP_CHUNK, (rows + 255) / 256 * 256);
+    *chunk = c;
+    return (int)((n_paths + c - 1) / c);
+}
 
 template <typename Real>
 hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, int group, hipStream_t s) {

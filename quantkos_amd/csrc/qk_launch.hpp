@@ -99,15 +99,17 @@ hipError_t launch_lsm_coop(const LsmCoopArgs<Real> &a, int n_basis, bool weighte
 // Blocks and paths per block of one request's pump work (group = threads per
 // block; at most max_blocks blocks, each a whole number of 256-path rows).
 int pump_blocks(int64_t n_paths, int group, int max_blocks, int64_t *chunk);
-// Block limit of a request in the server's pump: every block re-reads all of
-// its request's partials (nb^2 x 128 B per date), and the pump advances up to
-// PUMP_MAX_REQ requests per launch next to batch launches, so a request gets
-// about sqrt(n_paths / 32) blocks (config-4 burst rate +3 % over the limit of
-// 256 for every request).
-inline int shared_pump_max_blocks(int64_t n_paths) {
-    const int64_t s = (int64_t)std::ceil(std::sqrt((double)n_paths / 32.0));
-    return (int)std::max<int64_t>(1, std::min<int64_t>(s, PUMP_MAX_BLOCKS));
-}
+// Blocks of a request in the server's two-launch pump step (k_lsm_solve +
+// k_lsm_paths): chunks of SERVER_PUMP_CHUNK paths (one pass of 8 rows of 256
+// threads), larger when the request would need more than
+// SERVER_PUMP_MAX_BLOCKS blocks, so that the one solve block per request
+// reads at most 1,024 partials (128 KiB) per date.
+constexpr int64_t SERVER_PUMP_CHUNK = 2048;
+constexpr int64_t SERVER_PUMP_MAX_BLOCKS = 1024;
+constexpr int LSM_SOLVE_GROUP = 1024;   // threads of a k_lsm_solve block
+int server_pump_blocks(int64_t n_paths, int64_t *chunk);
+template <typename Real>
+hipError_t launch_pump_two_phase(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s);
 
 // A request of a serving batch (one payoff, or a path store for LSM).
 constexpr int32_t REQ_PATHDEP = 1, REQ_STORE = 2;

@@ -107,10 +107,21 @@ struct SReq {
     // American: pump state
     int t = -1;                    // next date to process
     int pslot = -1;                // pinned staging slot while in the pump
+    int pump = -1;                 // index of the pump chain that advances it
     hipEvent_t ev = nullptr;       // backward sweep and merge finished
     hipEvent_t ev_fwd = nullptr;   // path store generated
     char *base = nullptr;
     char *dev(size_t o) const { return base + off + o; }
+};
+
+// One Longstaff-Schwartz pump chain: its own stream, on which every step
+// advances each of its requests by one date.  Two chains run side by side so
+// that one chain's solve launch and launch gaps overlap the other's path work.
+constexpr int N_PUMPS = 4;
+struct PumpChain {
+    hipStream_t stream = nullptr;
+    std::vector<SReq *> active;            // in this chain (t >= 0)
+    std::deque<hipEvent_t> steps;          // events of queued steps
 };
 
 struct Slot {
@@ -135,13 +146,11 @@ struct qk_server {
     Arena arena;
     std::vector<Slot> slots;
     // Longstaff-Schwartz pump
-    hipStream_t pump_stream = nullptr;
-    char *pump_hbuf = nullptr;                 // [PUMP_MAX_REQ][PUMP_SLOT_BYTES] pinned
+    qk::PumpChain pumps[qk::N_PUMPS];
+    char *pump_hbuf = nullptr;                 // [N_PUMPS * PUMP_MAX_REQ][PUMP_SLOT_BYTES] pinned
     std::vector<bool> pslot_used;
-    std::deque<SReq *> lsm_waiting;            // forward launched, not yet in the pump
-    std::vector<SReq *> pump_active;           // in the pump (t >= 0)
+    std::deque<SReq *> lsm_waiting;            // forward launched, not yet in a pump
     std::vector<SReq *> lsm_finishing;         // merge enqueued, completion pending
-    std::deque<hipEvent_t> pump_steps;         // events of queued pump steps
     std::vector<hipEvent_t> event_pool;
     std::thread th;
     std::mutex mu;
@@ -382,12 +391,12 @@ void fill_pump_req(PumpReq<Real> &q, const SReq *r) {
 // request of that kind by one date (the basis is a template parameter of the
 // date kernel).
 template <typename Real>
-hipError_t launch_pump_step(qk_server *srv, bool fp32) {
+hipError_t launch_pump_step(PumpChain &pc, bool fp32) {
     for (int nbasis = 3; nbasis <= LSM_MAX_BASIS; ++nbasis) {
         for (int w = 0; w < 2; ++w) {
             PumpArgs<Real> a{};
             int blocks = 0;
-            for (SReq *r : srv->pump_active) {
+            for (SReq *r : pc.active) {
                 if ((r->c.precision == QK_FP32) != fp32 || r->n_basis != nbasis || (int)r->weighted != w) continue;
                 fill_pump_req<Real>(a.r[a.n_req], r);
                 a.block_start[a.n_req] = blocks;
@@ -396,7 +405,7 @@ hipError_t launch_pump_step(qk_server *srv, bool fp32) {
             }
             if (a.n_req == 0) continue;
             a.block_start[a.n_req] = blocks;
-            const hipError_t e = launch_pump<Real>(a, nbasis, w != 0, LSM_SHARED_GROUP, srv->pump_stream);
+            const hipError_t e = launch_pump_two_phase<Real>(a, nbasis, w != 0, pc.stream);
             if (e != hipSuccess) return e;
         }
     }
@@ -405,7 +414,7 @@ hipError_t launch_pump_step(qk_server *srv, bool fp32) {
 
 // Enqueues the merge of a request whose date-0 step was just queued.
 hipError_t enqueue_lsm_merge(qk_server *srv, SReq *r) {
-    hipStream_t s = srv->pump_stream;
+    hipStream_t s = srv->pumps[r->pump].stream;
     char *h = srv->pump_hbuf + (size_t)r->pslot * PUMP_SLOT_BYTES;
     const MergeReqD m = merge_desc(*r, 0, r->c.price_scale);     // no e^{-rT}: OptionPricer.hpp:211-216
     std::memcpy(h, &m, sizeof(m));
@@ -440,20 +449,25 @@ void finish_american(SReq *r, const qk_stats &st) {
     }
 }
 
-// Advances the pump: admits requests whose path store is ready, queues one
-// step when fewer than PUMP_DEPTH are in flight, and hands requests that
-// reached date 0 to the merge.  Returns true if anything was queued.
+// Advances the pumps: admits requests whose path store is ready to the chain
+// with the fewest paths in flight, queues one step on each chain with fewer
+// than PUMP_DEPTH steps in flight, and hands requests that reached date 0 to
+// the merge.  Returns true if anything was queued.
 bool pump(qk_server *srv) {
     bool progress = false;
-    while (!srv->pump_steps.empty() && hipEventQuery(srv->pump_steps.front()) != hipErrorNotReady) {
-        srv->event_pool.push_back(srv->pump_steps.front());
-        srv->pump_steps.pop_front();
+    int n_active = 0;
+    for (PumpChain &pc : srv->pumps) {
+        while (!pc.steps.empty() && hipEventQuery(pc.steps.front()) != hipErrorNotReady) {
+            srv->event_pool.push_back(pc.steps.front());
+            pc.steps.pop_front();
+        }
+        n_active += (int)pc.active.size();
     }
     for (auto it = srv->lsm_waiting.begin(); it != srv->lsm_waiting.end();) {
         // A request keeps its staging slot until its merge has completed, so
-        // the slots bound the pump and the finishing list together.
-        if ((int)srv->pump_active.size() >= srv->cfg.max_lsm_streams ||
-            (int)(srv->pump_active.size() + srv->lsm_finishing.size()) >= PUMP_MAX_REQ)
+        // the slots bound the pumps and the finishing list together.
+        if (n_active >= srv->cfg.max_lsm_streams ||
+            n_active + (int)srv->lsm_finishing.size() >= (int)srv->pslot_used.size())
             break;
         SReq *r = *it;
         const hipError_t q = hipEventQuery(r->ev_fwd);
@@ -467,6 +481,22 @@ bool pump(qk_server *srv) {
             busy_end(srv);
             continue;
         }
+        int best = -1;
+        int64_t best_paths = 0;
+        for (int k = 0; k < N_PUMPS; ++k) {
+            PumpChain &pc = srv->pumps[k];
+            if ((int)pc.active.size() >= PUMP_MAX_REQ) continue;
+            int64_t paths = 0;
+            for (const SReq *x : pc.active) paths += x->c.n_paths;
+            if (best < 0 || paths < best_paths) {
+                best = k;
+                best_paths = paths;
+            }
+        }
+        if (best < 0) {                                // every chain full: wait
+            srv->lsm_waiting.push_front(r);
+            break;
+        }
         for (size_t k = 0; k < srv->pslot_used.size(); ++k) {
             if (!srv->pslot_used[k]) {
                 srv->pslot_used[k] = true;
@@ -475,36 +505,41 @@ bool pump(qk_server *srv) {
             }
         }
         r->t = r->c.n_steps - 1;
-        srv->pump_active.push_back(r);
+        r->pump = best;
+        srv->pumps[best].active.push_back(r);
+        ++n_active;
         progress = true;
     }
-    if (srv->pump_active.empty() || (int)srv->pump_steps.size() >= PUMP_DEPTH) return progress;
-    hipError_t e = launch_pump_step<float>(srv, true);
-    if (e == hipSuccess) e = launch_pump_step<double>(srv, false);
-    std::vector<SReq *> keep;
-    for (SReq *r : srv->pump_active) {
-        if (e == hipSuccess && r->t == 0) {
-            e = enqueue_lsm_merge(srv, r);
-            if (e == hipSuccess) {
-                srv->lsm_finishing.push_back(r);
+    for (PumpChain &pc : srv->pumps) {
+        if (pc.active.empty() || (int)pc.steps.size() >= PUMP_DEPTH) continue;
+        hipError_t e = launch_pump_step<float>(pc, true);
+        if (e == hipSuccess) e = launch_pump_step<double>(pc, false);
+        std::vector<SReq *> keep;
+        for (SReq *r : pc.active) {
+            if (e == hipSuccess && r->t == 0) {
+                e = enqueue_lsm_merge(srv, r);
+                if (e == hipSuccess) {
+                    srv->lsm_finishing.push_back(r);
+                    continue;
+                }
+            }
+            if (e != hipSuccess) {
+                r->err = hipGetErrorString(e);
+                complete(srv, r, QK_EDEVICE);
+                busy_end(srv);
                 continue;
             }
+            --r->t;
+            keep.push_back(r);
         }
-        if (e != hipSuccess) {
-            r->err = hipGetErrorString(e);
-            complete(srv, r, QK_EDEVICE);
-            busy_end(srv);
-            continue;
+        pc.active.swap(keep);
+        if (hipEvent_t ev = take_event(srv)) {
+            if (hipEventRecord(ev, pc.stream) == hipSuccess) pc.steps.push_back(ev);
+            else srv->event_pool.push_back(ev);
         }
-        --r->t;
-        keep.push_back(r);
+        progress = true;
     }
-    srv->pump_active.swap(keep);
-    if (hipEvent_t ev = take_event(srv)) {
-        if (hipEventRecord(ev, srv->pump_stream) == hipSuccess) srv->pump_steps.push_back(ev);
-        else srv->event_pool.push_back(ev);
-    }
-    return true;
+    return progress;
 }
 
 // One pass of the scheduler: retire finished work, run the pump, then admit
@@ -629,7 +664,8 @@ extern "C" qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, 
     }
     if (!s->cfg.mem_budget_bytes) s->cfg.mem_budget_bytes = (size_t)(0.6 * (double)free_b);
     if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 3;
-    if (s->cfg.max_lsm_streams <= 0 || s->cfg.max_lsm_streams > PUMP_MAX_REQ) s->cfg.max_lsm_streams = PUMP_MAX_REQ;
+    if (s->cfg.max_lsm_streams <= 0 || s->cfg.max_lsm_streams > N_PUMPS * PUMP_MAX_REQ)
+        s->cfg.max_lsm_streams = N_PUMPS * PUMP_MAX_REQ;
     if (s->cfg.item_path_steps <= 0) s->cfg.item_path_steps = 131072;
     if (s->cfg.blocks_per_cu <= 0) s->cfg.blocks_per_cu = 8;
     int n_cu = 0;
@@ -647,9 +683,12 @@ extern "C" qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, 
             return fail(ctx, QK_EDEVICE, "stream creation failed");
         }
     }
-    s->pslot_used.assign(PUMP_MAX_REQ, false);
-    if (hipStreamCreateWithFlags(&s->pump_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void **)&s->pump_hbuf, PUMP_MAX_REQ * PUMP_SLOT_BYTES, hipHostMallocDefault) != hipSuccess) {
+    s->pslot_used.assign(N_PUMPS * PUMP_MAX_REQ, false);
+    bool pump_ok = hipHostMalloc((void **)&s->pump_hbuf, N_PUMPS * PUMP_MAX_REQ * PUMP_SLOT_BYTES,
+                                 hipHostMallocDefault) == hipSuccess;
+    for (PumpChain &pc : s->pumps)
+        pump_ok = pump_ok && hipStreamCreateWithFlags(&pc.stream, hipStreamNonBlocking) == hipSuccess;
+    if (!pump_ok) {
         qk_server_destroy(s);
         return fail(ctx, QK_EDEVICE, "pump stream creation failed");
     }
@@ -675,11 +714,13 @@ extern "C" qk_status qk_server_destroy(qk_server *s) {
         if (sl.hstats) (void)hipHostFree(sl.hstats);
         sl.dbuf.release();
     }
-    if (s->pump_stream) {
-        (void)hipStreamSynchronize(s->pump_stream);
-        (void)hipStreamDestroy(s->pump_stream);
+    for (PumpChain &pc : s->pumps) {
+        if (pc.stream) {
+            (void)hipStreamSynchronize(pc.stream);
+            (void)hipStreamDestroy(pc.stream);
+        }
+        for (hipEvent_t e : pc.steps) (void)hipEventDestroy(e);
     }
-    for (hipEvent_t e : s->pump_steps) (void)hipEventDestroy(e);
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
     if (s->pump_hbuf) (void)hipHostFree(s->pump_hbuf);
     for (auto &kv : s->reqs) {
@@ -724,7 +765,7 @@ extern "C" qk_status qk_server_submit_ex(qk_server *s, const qk_contract *c, con
     r->n_tiles = (c->n_paths + TILE - 1) / TILE;
     r->n_chunks = (r->n_tiles + MERGE_CHUNK - 1) / MERGE_CHUNK;
     if (american)
-        r->nb = pump_blocks(c->n_paths, LSM_SHARED_GROUP, shared_pump_max_blocks(c->n_paths), &r->chunk);
+        r->nb = server_pump_blocks(c->n_paths, &r->chunk);
     r->lay = layout_of(*c, american, r->n_tiles, r->n_chunks, r->nb);
     r->t_submit = clk::now();
     if (american && (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess ||
