@@ -24,14 +24,28 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
+// PLAIN: rounds written with ordinary XORs, which the compiler keeps on the
+// scalar unit when their operands are uniform.  With the counter (path low,
+// path high, block, slot), a uniform path-high word and a uniform block,
+// rounds 1 and 2 consist of products that are uniform or fixed per path
+// (hoisted out of the step loop) and XORs of them: ordinary XORs there cost
+// two vector instructions in all; bitop3 would force them onto the vector
+// unit.
+template <int PLAIN = 0>
 __device__ __forceinline__ Words philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                         uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+        uint32_t n0, n2;
+        if (r < PLAIN) {
+            n0 = (uint32_t)(p1 >> 32) ^ (c1 ^ k0);
+            n2 = (uint32_t)(p0 >> 32) ^ (c3 ^ k1);
+        } else {
+            n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+            n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+        }
         c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;

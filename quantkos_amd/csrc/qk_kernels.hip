@@ -37,7 +37,7 @@ namespace qk {
 // One tile (64 paths, one per lane) for L lanes sharing each step's Philox
 // block.  Lane l's moments for payoff j go to part_*[l*lane_stride + j*pay_stride
 // + out_index]; path p of the tile draws counter (path0 + p, step, slot).
-template <typename Real, int MODEL, int SCHEME, bool PATHDEP, bool STORE, int L>
+template <typename Real, int MODEL, int SCHEME, bool PATHDEP, bool STORE, int L, bool HIU = true>
 __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int n_valid, const PayD<Real> *__restrict__ pays,
                                          int n_pay, int n_steps, int64_t n_paths, int64_t path0, uint32_t key0,
                                          uint32_t key1, Real itm_eps, int64_t tile, Real *__restrict__ store,
@@ -49,7 +49,13 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
     const int64_t path = tile * TILE + lid;
     const bool active = path < n_paths;
     const uint64_t gpath = (uint64_t)(path0 + path);     // counter of the random stream
-    const uint32_t p0 = (uint32_t)gpath, p1 = (uint32_t)(gpath >> 32);
+    // HIU: the caller guarantees path0 % 64 == 0, so the tile's 64 paths share
+    // the high counter word; held in a scalar register it makes the first two
+    // Philox rounds' products uniform or loop-invariant (scalar unit, hoisted)
+    // instead of per-lane work at every step.
+    const uint32_t p0 = (uint32_t)gpath;
+    const uint32_t p1 = HIU ? __builtin_amdgcn_readfirstlane((uint32_t)(gpath >> 32)) : (uint32_t)(gpath >> 32);
+    constexpr int PH = HIU ? 2 : 0;
 
     // Lane constants held in registers for the whole path loop (scalar
     // registers: the lane block is uniform per wavefront).
@@ -106,12 +112,12 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
         // of block (path, i, slot 0).
         int t = 0;
         for (; t + 1 < n_steps; t += 2) {
-            const Words w = philox(p0, p1, (uint32_t)t >> 1, 0u, key0, key1);
+            const Words w = philox<PH>(p0, p1, (uint32_t)t >> 1, 0u, key0, key1);
             step(t, w.w0, w.w1, 0u, 0u);
             step(t + 1, w.w2, w.w3, 0u, 0u);
         }
         if (t < n_steps) {
-            const Words w = philox(p0, p1, (uint32_t)t >> 1, 0u, key0, key1);
+            const Words w = philox<PH>(p0, p1, (uint32_t)t >> 1, 0u, key0, key1);
             step(t, w.w0, w.w1, 0u, 0u);
         }
     } else if constexpr (MODEL == MODEL_HESTON && SCHEME == SCHEME_QE) {
@@ -127,12 +133,12 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
                 const uint32_t k = 3u * (uint32_t)t, r = k & 3u;
                 uint32_t a0, a1, a2;
                 if (r == 0u) {
-                    cur = philox(p0, p1, k >> 2, 0u, key0, key1);
+                    cur = philox<PH>(p0, p1, k >> 2, 0u, key0, key1);
                     a0 = cur.w0; a1 = cur.w1; a2 = cur.w2;
                 } else if (r == 1u) {
                     a0 = cur.w1; a1 = cur.w2; a2 = cur.w3;
                 } else {
-                    const Words nx = philox(p0, p1, (k >> 2) + 1u, 0u, key0, key1);
+                    const Words nx = philox<PH>(p0, p1, (k >> 2) + 1u, 0u, key0, key1);
                     if (r == 3u) {
                         a0 = cur.w3; a1 = nx.w0; a2 = nx.w1;
                     } else {
@@ -146,20 +152,20 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
             int t = 0;
             uint32_t b = 0;
             for (; t + 3 < n_steps; t += 4, b += 3) {
-                const Words x = philox(p0, p1, b, 0u, key0, key1);
+                const Words x = philox<PH>(p0, p1, b, 0u, key0, key1);
                 step(t, x.w0, x.w1, x.w2, 0u);
-                const Words y = philox(p0, p1, b + 1, 0u, key0, key1);
+                const Words y = philox<PH>(p0, p1, b + 1, 0u, key0, key1);
                 step(t + 1, x.w3, y.w0, y.w1, 0u);
-                const Words z = philox(p0, p1, b + 2, 0u, key0, key1);
+                const Words z = philox<PH>(p0, p1, b + 2, 0u, key0, key1);
                 step(t + 2, y.w2, y.w3, z.w0, 0u);
                 step(t + 3, z.w1, z.w2, z.w3, 0u);
             }
             for (; t < n_steps; ++t) {      // at most 3 steps; word 3t is word 0, 3 or 2 of its block
                 const uint32_t k = 3u * (uint32_t)t;
-                const Words x = philox(p0, p1, k >> 2, 0u, key0, key1);
+                const Words x = philox<PH>(p0, p1, k >> 2, 0u, key0, key1);
                 uint32_t a0 = x.w0, a1 = x.w1, a2 = x.w2;
                 if ((k & 3u) != 0u) {
-                    const Words y = philox(p0, p1, (k >> 2) + 1u, 0u, key0, key1);
+                    const Words y = philox<PH>(p0, p1, (k >> 2) + 1u, 0u, key0, key1);
                     if ((k & 3u) == 3u) {
                         a0 = x.w3; a1 = y.w0; a2 = y.w1;
                     } else {
@@ -171,7 +177,7 @@ __device__ __forceinline__ void run_tile(const LaneC<Real> *__restrict__ lc, int
         }
     } else {
         for (int t = 0; t < n_steps; ++t) {
-            const Words w = philox(p0, p1, (uint32_t)t, 0u, key0, key1);
+            const Words w = philox<PH>(p0, p1, (uint32_t)t, 0u, key0, key1);
             step(t, w.w0, w.w1, w.w2, w.w3);
         }
     }
@@ -209,7 +215,8 @@ __global__ __launch_bounds__(BLOCK) void k_paths(PathArgs<Real> a) {
     const int64_t tile = a.tile_off + (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x >> 6);
     if (tile >= a.n_tiles) return;                       // uniform per wavefront
     const int lbase = blockIdx.y * L;
-    run_tile<Real, MODEL, SCHEME, PATHDEP, STORE, L>(
+    // STORE kernels also serve windows of the stream at any path0
+    run_tile<Real, MODEL, SCHEME, PATHDEP, STORE, L, !STORE>(
         a.lanes + lbase, a.n_lanes - lbase, a.pays, a.n_pay, a.n_steps, a.n_paths, a.path0, a.key0, a.key1,
         a.itm_eps, tile, a.store, a.part_sum, a.part_m2, a.part_itm, (int64_t)lbase * a.n_pay * a.n_tiles + tile,
         (int64_t)a.n_pay * a.n_tiles, a.n_tiles, lbase == 0 ? a.pay_out : nullptr);

@@ -118,6 +118,8 @@ struct SReq {
 // advances each of its requests by one date.  Two chains run side by side so
 // that one chain's solve launch and launch gaps overlap the other's path work.
 constexpr int N_PUMPS = 4;
+// Pending requests examined behind one that waits for memory.
+constexpr int ADMIT_SCAN = 512;
 struct PumpChain {
     hipStream_t stream = nullptr;
     std::vector<SReq *> active;            // in this chain (t >= 0)
@@ -283,17 +285,20 @@ hipEvent_t take_event(qk_server *srv) {
 hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
     std::vector<ReqD<float>> r32;
     std::vector<ReqD<double>> r64;
-    std::vector<ItemD> i32, i64;
+    // Work items of American requests (path stores for the pump) and of the
+    // rest, launched in that order so that the pump can start on the
+    // American requests while the rest of the batch still runs.
+    std::vector<ItemD> i32, i64, a32, a64;
     std::vector<MergeReqD> mreq;
     std::vector<int32_t> chunk_req;
     slot.merged.clear();
     for (SReq *r : batch) {
         const double eps = r->c.precision == QK_FP32 ? 1e-6 : 1e-12;
         if (r->c.precision == QK_FP32) {
-            add_items(i32, (int)r32.size(), *r, srv->cfg.item_path_steps);
+            add_items(r->american ? a32 : i32, (int)r32.size(), *r, srv->cfg.item_path_steps);
             r32.push_back(make_req<float>(*r, eps));
         } else {
-            add_items(i64, (int)r64.size(), *r, srv->cfg.item_path_steps);
+            add_items(r->american ? a64 : i64, (int)r64.size(), *r, srv->cfg.item_path_steps);
             r64.push_back(make_req<double>(*r, eps));
         }
         if (!r->american) {
@@ -308,9 +313,10 @@ hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
     std::vector<char> buf;
     buf.reserve(4096);
     const size_t o_r32 = stage(buf, r32), o_r64 = stage(buf, r64), o_i32 = stage(buf, i32), o_i64 = stage(buf, i64);
+    const size_t o_a32 = stage(buf, a32), o_a64 = stage(buf, a64);
     const size_t o_m = stage(buf, mreq), o_c = stage(buf, chunk_req);
     const size_t o_cnt = up(buf.size());
-    buf.resize(o_cnt + 256, 0);                              // two batch counters, zeroed by the copy
+    buf.resize(o_cnt + 256, 0);                              // four batch counters, zeroed by the copy
     const size_t o_st = up(buf.size());
     buf.resize(o_st + mreq.size() * sizeof(qk_stats) + 8);
     hipError_t e;
@@ -341,16 +347,24 @@ hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
     hipStream_t s = slot.stream;
     const int blocks = srv->n_cu * srv->cfg.blocks_per_cu;
     if ((e = hipMemcpyAsync(d, slot.hbuf, o_st, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
-    const int b32 = (int)std::min<int64_t>(blocks, ((int64_t)i32.size() + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
-    const int b64 = (int)std::min<int64_t>(blocks, ((int64_t)i64.size() + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
-    if ((e = launch_batch<float>((const ReqD<float> *)(d + o_r32), (const ItemD *)(d + o_i32), (int32_t)i32.size(),
-                                 (unsigned int *)(d + o_cnt), b32, s)) != hipSuccess)
+    auto nblk = [&](size_t n_items) {
+        return (int)std::min<int64_t>(blocks, ((int64_t)n_items + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    };
+    // each launch has its own item counter (64 bytes apart)
+    if ((e = launch_batch<float>((const ReqD<float> *)(d + o_r32), (const ItemD *)(d + o_a32), (int32_t)a32.size(),
+                                 (unsigned int *)(d + o_cnt), nblk(a32.size()), s)) != hipSuccess)
         return e;
-    if ((e = launch_batch<double>((const ReqD<double> *)(d + o_r64), (const ItemD *)(d + o_i64), (int32_t)i64.size(),
-                                  (unsigned int *)(d + o_cnt + 128), b64, s)) != hipSuccess)
+    if ((e = launch_batch<double>((const ReqD<double> *)(d + o_r64), (const ItemD *)(d + o_a64), (int32_t)a64.size(),
+                                  (unsigned int *)(d + o_cnt + 64), nblk(a64.size()), s)) != hipSuccess)
         return e;
     for (SReq *r : batch)
         if (r->american && (e = hipEventRecord(r->ev_fwd, s)) != hipSuccess) return e;
+    if ((e = launch_batch<float>((const ReqD<float> *)(d + o_r32), (const ItemD *)(d + o_i32), (int32_t)i32.size(),
+                                 (unsigned int *)(d + o_cnt + 128), nblk(i32.size()), s)) != hipSuccess)
+        return e;
+    if ((e = launch_batch<double>((const ReqD<double> *)(d + o_r64), (const ItemD *)(d + o_i64), (int32_t)i64.size(),
+                                  (unsigned int *)(d + o_cnt + 192), nblk(i64.size()), s)) != hipSuccess)
+        return e;
     if (!mreq.empty()) {
         if ((e = launch_batch_merge((const MergeReqD *)(d + o_m), (int)mreq.size(), (const int32_t *)(d + o_c),
                                     (int64_t)chunk_req.size(), (qk_stats *)(d + o_st), s)) != hipSuccess)
@@ -579,8 +593,12 @@ bool schedule_once(qk_server *srv) {
     if (pump(srv)) progress = true;
     if (progress) srv->cv_done.notify_all();
     // Admission in arrival order while memory and a batch slot are free.  A
-    // request that does not fit the free memory stops admission, so large
-    // requests are not starved by smaller ones.
+    // request that does not fit the free memory stops admission of requests
+    // of its size, so large requests are not starved by smaller ones; small
+    // requests behind it (no path store, at most 1/64 of the arena: tile
+    // moments only, released when their batch ends) may still go, so the
+    // GPU keeps computing while the blocked request waits for memory that
+    // the Longstaff-Schwartz pump is freeing.
     Slot *free_slot = nullptr;
     for (Slot &slot : srv->slots)
         if (!slot.busy) {
@@ -589,24 +607,37 @@ bool schedule_once(qk_server *srv) {
         }
     if (!free_slot || srv->pending.empty()) return progress;
     std::vector<SReq *> batch;
-    while (!srv->pending.empty()) {
-        SReq *r = srv->pending.front();
+    bool blocked = false;
+    int scanned = 0;
+    for (auto it = srv->pending.begin(); it != srv->pending.end() && scanned < ADMIT_SCAN;) {
+        SReq *r = *it;
         if (r->lay.total > srv->arena.size) {
-            srv->pending.pop_front();
+            it = srv->pending.erase(it);
             r->err = "request footprint exceeds the server arena";
             complete(srv, r, QK_ENOMEM);
             srv->cv_done.notify_all();
             progress = true;
             continue;
         }
+        if (blocked) {
+            ++scanned;
+            if (r->american || r->lay.total > srv->arena.size / 64) {
+                ++it;
+                continue;
+            }
+        }
         const size_t off = srv->arena.alloc(r->lay.total);
-        if (off == SIZE_MAX) break;
+        if (off == SIZE_MAX) {
+            blocked = true;
+            ++it;
+            continue;
+        }
         r->off = off;
         r->base = srv->arena.base;
         r->t_admit = clk::now();
         r->admitted = true;
         batch.push_back(r);
-        srv->pending.pop_front();
+        it = srv->pending.erase(it);
     }
     if (batch.empty()) return progress;
     const hipError_t e = launch_slot(srv, *free_slot, batch);
