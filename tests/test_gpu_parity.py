@@ -423,6 +423,29 @@ def test_config0_contract(engine, oracle):
     assert abs(q.price - cf) <= 3.0 * q.se, (q.price, cf, q.se)
 
 
+def test_config0_euler_bias_at_64m_paths(engine, oracle):
+    """The same Euler contract at 64M paths for three seeds, FP64 and FP32: the price sits
+    +0.032 above the characteristic-function price (the discretisation bias of the
+    reference's absorbed variance at 252 steps with the Feller condition violated,
+    2*kappa*theta = 0.16 < sigma^2 = 0.25), far outside the noise (pooled SE ~0.0008),
+    and FP32 agrees with FP64 on the same random numbers within 0.05 SE."""
+    kw = dict(model=qk.HESTON, scheme=qk.EULER, n_steps=252, S0=1.0, v0=0.04, r=0.03, q=0.0,
+              T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7, price_scale=100.0)
+    p = qe.payoff(qk.EUROPEAN, qk.CALL, 1.0)
+    cf = 100.0 * oracle.heston_call(1.0, 1.0, 1.0, 0.03, 0.0, 0.04, 2.0, 0.04, 0.5, -0.7)
+    bias, var = [], []
+    for seed in (1, 2, 3):
+        r64 = engine.price(qe.contract(precision=qk.FP64, n_paths=64_000_000, seed=seed, **kw), p)
+        r32 = engine.price(qe.contract(precision=qk.FP32, n_paths=64_000_000, seed=seed, **kw), p)
+        assert abs(r32.price - r64.price) <= 0.05 * r64.se, (seed, r32.price, r64.price, r64.se)
+        bias.append(r64.price - cf)
+        var.append(r64.se ** 2)
+    b = sum(bias) / 3
+    se = math.sqrt(sum(var)) / 3
+    assert b > 20 * se, (bias, se)
+    assert abs(b - 0.032) <= 0.001 + 4 * se, (bias, se)
+
+
 @pytest.mark.parametrize("basis,weighted", [(qk.LAGUERRE_P02, False), (qk.LAGUERRE_P04, False),
                                             (qk.LAGUERRE_P03, True)])
 def test_american_other_bases_match_oracle(engine, oracle, basis, weighted):
