@@ -153,6 +153,36 @@ qk_status qk_price_payoffs(qk_ctx *ctx, const qk_contract *c, const qk_payoff *p
 qk_status qk_price_payoffs_noalloc(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoffs,
                                    int32_t n_payoffs, qk_stats *out);
 
+/* ---- batches: path windows priced separately, merged on the device ----
+ * The reference prices a path set in batches of MC.batch_size paths
+ * (PathsMCBatchMem.hpp:293-338, run_forward_all_mc_batches in
+ * MCEngine.hpp:489-520: MSolver::execute_batch per batch, then
+ * OptionPricer::accumulate_batch_payoffs, OptionPricer.hpp:123-193).  Here a
+ * batch is a window of the path set: qk_price_window returns the window's
+ * merge chunks (the fixed-order merge's first stage, one per 65,536 paths of
+ * the window) and qk_merge_windows runs the second stage over the chunks of
+ * every window.  When every window starts at a multiple of
+ * QK_WINDOW_CHUNK_PATHS, the merged result equals qk_price_payoffs over the
+ * whole path set bit for bit; other 64-aligned windows merge exactly, in a
+ * different summation order. */
+#define QK_WINDOW_CHUNK_PATHS ((int64_t)65536)
+/* Merge state of a run of paths: count, mean and sum of squared deviations of
+ * the undiscounted payoff (before the price scale), in-the-money count. */
+typedef struct { double n, mean, m2; uint64_t itm; } qk_moments;
+
+/* Prices paths [path0, path0 + c->n_paths) of the path set of (c->seed, the
+ * contract) -- the paths a single launch would number path0 .. -- and writes
+ * chunks[j * nc + k] for payoff j and chunk k, nc = ceil(c->n_paths / 65536).
+ * path0 must be a multiple of 64. */
+qk_status qk_price_window(qk_ctx *ctx, const qk_contract *c, int64_t path0, const qk_payoff *payoffs,
+                          int32_t n_payoffs, qk_moments *chunks);
+
+/* Merges chunks[j * n_chunks + k] (the windows' chunks in path order for each
+ * payoff j) and finalises price, SE and P(ITM) with the discount e^{-rT} and
+ * the price scale of c: out[n_payoffs]. */
+qk_status qk_merge_windows(qk_ctx *ctx, const qk_contract *c, const qk_payoff *payoffs, int32_t n_payoffs,
+                           const qk_moments *chunks, int64_t n_chunks, qk_stats *out);
+
 /* Lanes = contracts that share model, scheme, precision, n_steps, n_paths and
  * seed, and therefore the random stream (common random numbers); they may
  * differ in S0, v0, r, q, T and the Heston/jump parameters.  One launch

@@ -21,6 +21,7 @@ FP32, FP64 = 0, 1
 LAGUERRE_P02, LAGUERRE_P03, LAGUERRE_P04 = 0, 1, 2
 LSM_AUTO, LSM_PER_DATE, LSM_COOPERATIVE = 0, 1, 2
 NOALLOC_SCRATCH_BYTES = 16 << 20   # QK_NOALLOC_SCRATCH_BYTES
+WINDOW_CHUNK_PATHS = 65536         # QK_WINDOW_CHUNK_PATHS
 
 
 class Contract(C.Structure):
@@ -39,6 +40,10 @@ class Payoff(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("price", C.c_double), ("se", C.c_double), ("prob_itm", C.c_double),
                 ("mean", C.c_double), ("m2", C.c_double), ("n", C.c_int64), ("itm", C.c_int64)]
+
+
+class Moments(C.Structure):
+    _fields_ = [("n", C.c_double), ("mean", C.c_double), ("m2", C.c_double), ("itm", C.c_uint64)]
 
 
 class Bumps(C.Structure):
@@ -137,6 +142,8 @@ def load():
         "qk_price": ([vp, P(Contract), P(Payoff), P(Stats)], st),
         "qk_price_payoffs": ([vp, P(Contract), P(Payoff), C.c_int32, P(Stats)], st),
         "qk_price_payoffs_noalloc": ([vp, P(Contract), P(Payoff), C.c_int32, P(Stats)], st),
+        "qk_price_window": ([vp, P(Contract), C.c_int64, P(Payoff), C.c_int32, P(Moments)], st),
+        "qk_merge_windows": ([vp, P(Contract), P(Payoff), C.c_int32, P(Moments), C.c_int64, P(Stats)], st),
         "qk_price_lanes": ([vp, P(Contract), C.c_int32, P(Payoff), C.c_int32, P(Stats)], st),
         "qk_price_greeks": ([vp, P(Contract), P(Payoff), C.c_int32, P(Bumps), P(Greeks)], st),
         "qk_price_american": ([vp, P(Contract), C.c_int32, C.c_double, P(Stats), P(C.c_double)], st),

@@ -410,9 +410,13 @@ struct LaneBufs {
 
 // Launches the path kernel(s) and the merge of a lane call on ctx->stream,
 // once the constants block is on the device (or its copy is queued before).
+// path0 numbers the first path in the random stream (a multiple of 64); with
+// chunks_only the merge stops after its first stage, leaving the chunk moments
+// [lane * n_pay + payoff][chunk] in B.mscratch.
 template <typename Real>
 static qk_status launch_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
-                              const ConstLayout &lay, const LaneBufs &B, Real *pay_out, Real *store = nullptr) {
+                              const ConstLayout &lay, const LaneBufs &B, Real *pay_out, Real *store = nullptr,
+                              int64_t path0 = 0, bool chunks_only = false) {
     const qk_contract &c0 = lanes[0];
     const bool pathdep = std::any_of(pays, pays + n_pay, [](const qk_payoff &p) { return path_dependent(p.type); });
     const int L = qk_lanes_per_thread(ctx, c0.precision, c0.scheme, n_lanes);
@@ -426,7 +430,7 @@ static qk_status launch_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes
     a.n_steps = c0.n_steps;
     a.n_paths = c0.n_paths;
     a.n_tiles = n_tiles;
-    a.path0 = 0;
+    a.path0 = path0;
     a.key0 = (uint32_t)c0.seed;
     a.key1 = (uint32_t)(c0.seed >> 32);
     a.itm_eps = (Real)(c0.precision == QK_FP32 ? 1e-6 : 1e-12);   // OptionPricer.hpp:316
@@ -462,7 +466,7 @@ static qk_status launch_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes
     QK_HIP(hipEventRecord(ctx->ev[1], s));
     QK_HIP(launch_merge(a.part_sum, a.part_m2, a.part_itm, n_tiles, c0.n_paths, n_lanes, n_pay,
                         (const double *)(B.consts + lay.o_disc), (const double *)(B.consts + lay.o_scale), B.mscratch,
-                        B.stats, s));
+                        B.stats, s, chunks_only));
     QK_HIP(hipEventRecord(ctx->ev[2], s));
     return QK_OK;
 }
@@ -472,7 +476,8 @@ static qk_status launch_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes
 // ctx->stats on the device.
 template <typename Real>
 static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, const qk_payoff *pays, int n_pay,
-                           Real *pay_out = nullptr, Real *store = nullptr) {
+                           Real *pay_out = nullptr, Real *store = nullptr, int64_t path0 = 0,
+                           bool chunks_only = false) {
     const ConstLayout lay = const_layout<Real>(n_lanes, n_pay);
     const int64_t n_tiles = (lanes[0].n_paths + TILE - 1) / TILE;
     const size_t groups = (size_t)n_lanes * n_pay;
@@ -491,7 +496,7 @@ static qk_status run_lanes(qk_ctx *ctx, const qk_contract *lanes, int n_lanes, c
     QK_HIP(hipMemcpyAsync(ctx->lanes.p, ctx->hstage, lay.bytes, hipMemcpyHostToDevice, ctx->stream));
     const LaneBufs B{(char *)ctx->lanes.p, (double *)ctx->psum.p, (double *)ctx->pm2.p, (uint32_t *)ctx->pitm.p,
                      ctx->mscratch.p, (qk_stats *)ctx->stats.p};
-    return launch_lanes<Real>(ctx, lanes, n_lanes, pays, n_pay, lay, B, pay_out, store);
+    return launch_lanes<Real>(ctx, lanes, n_lanes, pays, n_pay, lay, B, pay_out, store, path0, chunks_only);
 }
 
 static qk_status finish_timing(qk_ctx *ctx, int64_t path_steps, bool has_lsm) {
@@ -579,6 +584,64 @@ extern "C" qk_status qk_price_payoffs(qk_ctx *ctx, const qk_contract *c, const q
 
 extern "C" qk_status qk_price(qk_ctx *ctx, const qk_contract *c, const qk_payoff *p, qk_stats *out) {
     return qk_price_lanes(ctx, c, 1, p, 1, out);
+}
+
+// ---------------------------------------------------------------------------
+// Batches as path windows (run_forward_all_mc_batches, MCEngine.hpp:489-520):
+// a window's first-stage merge chunks out, the second stage over every
+// window's chunks on the device.
+// ---------------------------------------------------------------------------
+static_assert(sizeof(qk_moments) == MERGE_MOM_BYTES, "qk_moments is the merge chunk record");
+static_assert(QK_WINDOW_CHUNK_PATHS == (int64_t)MERGE_CHUNK * TILE, "a merge chunk holds 65,536 paths");
+
+extern "C" qk_status qk_price_window(qk_ctx *ctx, const qk_contract *c, int64_t path0, const qk_payoff *pays,
+                                     int32_t n_pay, qk_moments *chunks) {
+    if (!ctx || !chunks) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_lanes(ctx, c, 1, pays, n_pay);
+    if (st != QK_OK) return st;
+    if (path0 < 0 || path0 % TILE != 0) return fail(ctx, QK_EINVAL, "path0 must be a non-negative multiple of 64");
+    if (path0 > MAX_PATH_STEPS - c->n_paths) return fail(ctx, QK_EINVAL, "path0 + n_paths must be <= 2^56");
+    if ((st = check_budget(ctx, footprint(*c, 1, n_pay, false))) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    st = c->precision == QK_FP32 ? run_lanes<float>(ctx, c, 1, pays, n_pay, nullptr, nullptr, path0, true)
+                                 : run_lanes<double>(ctx, c, 1, pays, n_pay, nullptr, nullptr, path0, true);
+    if (st != QK_OK) return st;
+    const int64_t n_tiles = (c->n_paths + TILE - 1) / TILE;
+    const size_t bytes = (size_t)n_pay * (size_t)merge_chunks(n_tiles) * MERGE_MOM_BYTES;
+    QK_HIP(hipMemcpyAsync(chunks, ctx->mscratch.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, false);
+}
+
+extern "C" qk_status qk_merge_windows(qk_ctx *ctx, const qk_contract *c, const qk_payoff *pays, int32_t n_pay,
+                                      const qk_moments *chunks, int64_t n_chunks, qk_stats *out) {
+    if (!ctx || !chunks || !out) return fail(ctx, QK_EINVAL, "null argument");
+    qk_status st = validate_lanes(ctx, c, 1, pays, n_pay);
+    if (st != QK_OK) return st;
+    if (n_chunks < 1 || n_chunks > ((int64_t)1 << 40) / n_pay)
+        return fail(ctx, QK_EINVAL, "n_chunks must be in [1, 2^40 / n_payoffs]");
+    const size_t cbytes = (size_t)n_pay * (size_t)n_chunks * MERGE_MOM_BYTES;
+    if ((st = check_budget(ctx, cbytes + 4096 + sizeof(qk_stats) * n_pay)) != QK_OK) return st;
+    QK_HIP(hipSetDevice(ctx->device));
+    // The discount and price scales as the lane call computes them.
+    const ConstLayout lay = c->precision == QK_FP32 ? const_layout<float>(1, n_pay) : const_layout<double>(1, n_pay);
+    if ((st = host_stage(ctx, std::max(lay.bytes, sizeof(qk_stats) * n_pay))) != QK_OK) return st;
+    if ((st = ctx->lanes.reserve(ctx, lay.bytes)) != QK_OK) return st;
+    if ((st = ctx->mscratch.reserve(ctx, cbytes)) != QK_OK) return st;
+    if ((st = ctx->stats.reserve(ctx, sizeof(qk_stats) * n_pay)) != QK_OK) return st;
+    QK_HIP(hipEventSynchronize(ctx->ev[0]));
+    if (c->precision == QK_FP32) pack_consts<float>(c, 1, pays, n_pay, lay, (char *)ctx->hstage);
+    else pack_consts<double>(c, 1, pays, n_pay, lay, (char *)ctx->hstage);
+    QK_HIP(hipMemcpyAsync(ctx->lanes.p, ctx->hstage, lay.bytes, hipMemcpyHostToDevice, ctx->stream));
+    QK_HIP(hipMemcpyAsync(ctx->mscratch.p, chunks, cbytes, hipMemcpyHostToDevice, ctx->stream));
+    QK_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    const char *cb = (const char *)ctx->lanes.p;
+    QK_HIP(launch_merge_final(ctx->mscratch.p, n_chunks, n_pay, n_pay, (const double *)(cb + lay.o_disc),
+                              (const double *)(cb + lay.o_scale), (qk_stats *)ctx->stats.p, ctx->stream));
+    QK_HIP(hipMemcpyAsync(ctx->hstage, ctx->stats.p, sizeof(qk_stats) * n_pay, hipMemcpyDeviceToHost, ctx->stream));
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out, ctx->hstage, sizeof(qk_stats) * n_pay);
+    return QK_OK;
 }
 
 // ---------------------------------------------------------------------------

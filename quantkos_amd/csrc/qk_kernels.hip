@@ -1032,14 +1032,21 @@ int64_t merge_chunks(int64_t n_tiles) { return (n_tiles + MERGE_CHUNK - 1) / MER
 
 hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pitm, int64_t n_tiles, int64_t n_paths,
                         int n_lanes, int n_pay, const double *disc, const double *scale, void *scratch, qk_stats *out,
-                        hipStream_t s) {
+                        hipStream_t s, bool chunks_only) {
     const int64_t nc = merge_chunks(n_tiles);
     const int groups = n_lanes * n_pay;
     hipLaunchKernelGGL(k_merge_chunks, dim3((unsigned)nc, (unsigned)groups), dim3(256), 0, s, ps, pm2, pitm, n_tiles,
-                       n_paths, nc, (Mom *)scratch, n_pay, disc, scale, nc == 1 ? out : nullptr);
-    if (nc > 1)
+                       n_paths, nc, (Mom *)scratch, n_pay, disc, scale, nc == 1 && !chunks_only ? out : nullptr);
+    if (nc > 1 && !chunks_only)
         hipLaunchKernelGGL(k_merge_final, dim3(groups), dim3(256), 0, s, (const Mom *)scratch, nc, n_pay, disc, scale,
                            out);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_final(const void *chunks, int64_t n_chunks, int groups, int n_pay, const double *disc,
+                              const double *scale, qk_stats *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_merge_final, dim3(groups), dim3(256), 0, s, (const Mom *)chunks, n_chunks, n_pay, disc, scale,
+                       out);
     return hipGetLastError();
 }
 

@@ -152,12 +152,18 @@ hipError_t launch_transpose_paths(const Real *store, int64_t n_paths, int32_t n_
 template <typename Real>
 hipError_t launch_paths(const PathArgs<Real> &a, int model, int scheme, bool pathdep, bool store, int L,
                         hipStream_t s);
-// Bytes of merge scratch for n_tiles tiles and `groups` (lane, payoff) pairs.
+// First-stage merge chunks of n_tiles tiles (MERGE_MOM_BYTES each per
+// (lane, payoff) pair).
 int64_t merge_chunks(int64_t n_tiles);
 constexpr size_t MERGE_MOM_BYTES = 32;
+// Both merge stages and the finalisation into out; with chunks_only, only the
+// first stage, whose chunk moments stay in scratch [group][chunk].
 hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pitm, int64_t n_tiles, int64_t n_paths,
                         int n_lanes, int n_pay, const double *disc, const double *scale, void *scratch, qk_stats *out,
-                        hipStream_t s);
+                        hipStream_t s, bool chunks_only = false);
+// The second stage and finalisation over chunk moments [group][n_chunks].
+hipError_t launch_merge_final(const void *chunks, int64_t n_chunks, int groups, int n_pay, const double *disc,
+                              const double *scale, qk_stats *out, hipStream_t s);
 hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s);
 // One k_lsm_step launch; group = threads per block, lsm_group<Real>() or
 // LSM_SHARED_GROUP (the blocks of every request must come from pump_blocks

@@ -155,6 +155,26 @@ class Engine:
         self._check(self.L.qk_price_payoffs_noalloc(self.h, C.byref(c), arr, len(payoffs), out))
         return list(out)
 
+    def price_window(self, c: Contract, path0: int, payoffs) -> list:
+        """Paths [path0, path0 + c.n_paths) of the path set: per payoff, the
+        window's merge chunks (one Moments per 65,536 paths)."""
+        nc = -(-c.n_paths // _lib.WINDOW_CHUNK_PATHS)
+        arr = (Payoff * len(payoffs))(*payoffs)
+        out = (_lib.Moments * (len(payoffs) * nc))()
+        self._check(self.L.qk_price_window(self.h, C.byref(c), path0, arr, len(payoffs), out))
+        return [list(out[j * nc:(j + 1) * nc]) for j in range(len(payoffs))]
+
+    def merge_windows(self, c: Contract, payoffs, chunks) -> list:
+        """Merges per-payoff chunk lists (windows concatenated in path order)
+        into one Stats per payoff, discounted and scaled as c prices."""
+        n = len(chunks[0])
+        assert all(len(x) == n for x in chunks) and len(chunks) == len(payoffs)
+        arr = (Payoff * len(payoffs))(*payoffs)
+        flat = (_lib.Moments * (len(payoffs) * n))(*[m for x in chunks for m in x])
+        out = (Stats * len(payoffs))()
+        self._check(self.L.qk_merge_windows(self.h, C.byref(c), arr, len(payoffs), flat, n, out))
+        return list(out)
+
     def price_lanes(self, lanes, payoffs) -> list:
         la = (Contract * len(lanes))(*lanes)
         pa = (Payoff * len(payoffs))(*payoffs)

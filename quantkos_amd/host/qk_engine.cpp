@@ -1,5 +1,6 @@
 // qk_engine.cpp -- the engine layer over the C ABI (see qk_engine.hpp).
 #include "qk_engine.hpp"
+#include "qk_solver.hpp"
 
 #include <cstdlib>
 #include <iomanip>
@@ -136,6 +137,7 @@ void MCGreeks::print_summary() const {
 // applied the discount factor (none for American) and the price scale.
 void OptionPricer::evaluate_option_price() {
     if (config.mc.N_Paths <= 0) throw std::runtime_error("Cannot compute metrics: Invalid number of paths.");
+    merge_batches();
     metrics.option_price = stats.price;
     metrics.standard_error = stats.se;
     metrics.prob_itm = stats.prob_itm;
@@ -232,6 +234,10 @@ OptionPricer::MCOpPrices price_forward(const KC::UInputs &scaled, bool print, bo
         // the reference's StorePaths kernel when an output file is named (MSolver.hpp:117-119)
         check(qk_price_store_paths(ctx, &c, &p, &s), "qk_price_store_paths");
         if (stored) *stored = true;
+    } else if (scaled.mc.batch_size > 0 && scaled.mc.batch_size < scaled.mc.N_Paths) {
+        // MC.batch_size: the reference's batch loop (ForwardMC.hpp:113-135), each
+        // batch a window of the path set; the same price as one launch
+        s = price_in_batches(scaled);
     } else {
         check(qk_price(ctx, &c, &p, &s), "qk_price");
     }

@@ -54,6 +54,8 @@ struct MCGreeks {
     void print_summary() const;
 };
 
+class PathsMCBatchMem;
+
 class OptionPricer {
 public:
     struct MCOpPrices {
@@ -79,12 +81,20 @@ public:
         dist = d;
         has_dist = true;
     }
+    // Batch by batch (OptionPricer.hpp:123-193): the batches' device merge
+    // chunks accumulate in BatchMem; evaluate_option_price merges them on the
+    // device (qk_merge_windows) once they cover N_Paths.
+    void accumulate_batch_payoffs(const PathsMCBatchMem &BatchMem, int64_t curr_batch_size);
     void evaluate_option_price();
     void print_info_option_price() const;
     const MCOpPrices &get_option_price_data() const;
+    const qk_stats &device_stats() const { return stats; }
 
 private:
+    void merge_batches();
     const KC::UInputs &config;
+    const PathsMCBatchMem *batch_mem = nullptr;
+    int64_t batch_paths = 0;
     qk_stats stats{};
     qk_distribution dist{};
     bool has_dist = false;

@@ -187,3 +187,41 @@ def test_american_yaml_writes_paths_and_european_reports_cf_price(in_tmp):
     want = qk.analytic_european(c, spec.payoff.right, spec.payoff.K)
     assert r.analytic_price == pytest.approx(want, rel=1e-12)
     assert r.analytic_err_se == pytest.approx((r.stats.price - want) / r.stats.se, rel=1e-9)
+
+
+def batch_yaml(in_tmp, batch_size, name):
+    text = open(os.path.join(GOLDEN, "bench_asian_qe.yaml")).read()
+    text = text.replace("N_Paths: 5000000", "N_Paths: 300000").replace("Batch_Size: 0", "Batch_Size: %d" % batch_size)
+    text = text.replace("Inp_DT: 0.0027397260273972603", "Inp_DT: 0.019230769230769232")
+    path = os.path.join(in_tmp, name)
+    open(path, "w").write(text)
+    return path
+
+
+def test_batch_size_prices_in_windows_bit_for_bit(in_tmp):
+    """MC.Batch_Size 100,000 (rounded up to whole 65,536-path merge chunks:
+    three batches of 131,072, 131,072 and 37,856) through the reference's batch
+    loop on the device windows gives the one-launch price bit for bit; the
+    C++ drop-in types (MSolver, RNGManager, PathsMCBatchMem, OptionPricer,
+    LSMEngine) written as the reference's runner does give the same numbers."""
+    batched, whole = batch_yaml(in_tmp, 100_000, "b.yaml"), batch_yaml(in_tmp, -1, "w.yaml")
+    r1, r0 = run_config(batched), run_config(whole)
+    assert (r1.stats.price, r1.stats.se, r1.stats.itm, r1.stats.n) == (r0.stats.price, r0.stats.se, r0.stats.itm,
+                                                                      r0.stats.n)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pkg = os.path.dirname(qk.__file__)
+    exe = str(in_tmp / "batch_solver")
+    subprocess.run(["g++", "-O2", "-std=c++20", "-I" + os.path.join(root, "include"), "-I" + os.path.join(pkg, "host"),
+                    os.path.join(root, "tests", "cpp", "batch_solver.cpp"), "-o", exe, "-L" + pkg, "-lquantkos",
+                    "-Wl,-rpath," + pkg], check=True, timeout=120)
+    out = subprocess.run([exe, batched], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = dict((l.split()[0], l.split()[1:]) for l in out.stdout.splitlines()
+                 if l.split() and l.split()[0] in ("batches", "asian", "american"))
+    assert lines["batches"] == ["3", "of", "131072"]
+    price, se, itm, n = lines["asian"]
+    assert (float(price), float(se), int(itm), int(n)) == (r0.stats.price, r0.stats.se, r0.stats.itm, r0.stats.n)
+    spec = load_config(batched)
+    with qk.Engine(0) as e:
+        a = e.price_american(spec.contract, qk.PUT, spec.payoff.K)
+    assert float(lines["american"][0]) == a.price and lines["american"][3] == "4"

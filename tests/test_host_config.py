@@ -153,3 +153,18 @@ def test_lsm_basis_and_distribution_keys(in_tmp):
     s = load_config(write(in_tmp, mc(BASE, "LSM_Basis: LaguerreP04\n  LSM_Weighted_Basis: true\n"
                                            "  analyze_risk_neutral_payoff_distribution: true")))
     assert (s.lsm.basis, s.lsm.weighted, s.analyze_distribution) == (qk.LAGUERRE_P04, 1, 1)
+
+
+def test_batch_solver_templates_compile(tmp_path):
+    """The drop-in batch types (MSolver, RNGManager, PathsMCBatchMem,
+    run_forward_all_mc_batches, LSMEngine) instantiate and link against
+    libquantkos as the reference's runner uses them (tests/cpp/batch_solver.cpp)."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pkg = os.path.dirname(qk.__file__)
+    exe = str(tmp_path / "batch_solver")
+    r = subprocess.run(["g++", "-O1", "-std=c++20", "-Wall", "-Werror", "-I" + os.path.join(root, "include"),
+                        "-I" + os.path.join(pkg, "host"), os.path.join(root, "tests", "cpp", "batch_solver.cpp"),
+                        "-o", exe, "-L" + pkg, "-lquantkos", "-Wl,-rpath," + pkg],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(exe)
