@@ -242,12 +242,14 @@ size_t footprint(const qk_contract &c, int n_lanes, int n_pay, bool american) {
         b = sat_add(b, sat_mul(sat_mul((size_t)c.n_paths, (size_t)c.n_steps), rs));   // path store [date][path]
         b = sat_add(b, sat_mul((size_t)c.n_paths, rs));                                // cash flows
         int64_t chunk = 0;
-        // Gram partials: as many as the larger of the plain call's and the
-        // request server's block counts.
+        // Right-hand-side partials: as many as the larger of the plain call's
+        // and the request server's block counts; the Gram pre-pass partials
+        // and factor records; the coefficients.
         b = sat_add(b, sat_mul((size_t)c.n_steps, LSM_MAX_BASIS * 8));
         const int nb = std::max(pump_blocks(c.n_paths, LSM_SHARED_GROUP, PUMP_MAX_BLOCKS, &chunk),
                                 server_pump_blocks(c.n_paths, &chunk));
         b = sat_add(b, (size_t)2 * nb * LSM_STRIDE * 8);
+        b = sat_add(b, lsm_prepass_bytes(c.n_paths, c.n_steps));
     }
     return b;
 }
@@ -306,7 +308,7 @@ extern "C" qk_status qk_ctx_create(int device, size_t mem_budget_bytes, qk_ctx *
 qk_ctx::~qk_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (DevBuf *b : {&lanes, &pays, &psum, &pm2, &pitm, &disc, &scale, &stats, &greeks, &store, &cf, &coeffs,
-                      &bpart, &counter, &sums, &mscratch, &payv, &hist, &tpose})
+                      &bpart, &counter, &sums, &mscratch, &payv, &hist, &tpose, &gpart, &factor})
         b->release();
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
@@ -902,8 +904,15 @@ static qk_status generate_store(qk_ctx *ctx, const qk_contract &c, int64_t path0
 template <typename Real>
 static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, double K, qk_stats *out,
                               double *coeffs_out, int n_basis = 4, bool weighted = false) {
-    qk_status st = generate_store<Real>(ctx, c);
-    if (st != QK_OK) return st;
+    // The merge's discount (1: no e^{-rT}, OptionPricer.hpp:211-216) and price
+    // scale, in one copy from the pinned staging buffer ahead of the paths.
+    qk_status st;
+    if ((st = ctx->disc.reserve(ctx, 16)) != QK_OK) return st;
+    if ((st = host_stage(ctx, 16)) != QK_OK) return st;
+    static_cast<double *>(ctx->hstage)[0] = 1.0;
+    static_cast<double *>(ctx->hstage)[1] = c.price_scale;
+    QK_HIP(hipMemcpyAsync(ctx->disc.p, ctx->hstage, 16, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = generate_store<Real>(ctx, c)) != QK_OK) return st;
     const int64_t n = c.n_paths;
     const int N = c.n_steps;
     const int64_t n_tiles = (n + TILE - 1) / TILE;
@@ -912,23 +921,36 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     if ((st = ctx->cf.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
     if ((st = ctx->coeffs.reserve(ctx, (size_t)N * n_basis * 8)) != QK_OK) return st;
     if ((st = ctx->bpart.reserve(ctx, (size_t)2 * nb * LSM_STRIDE * 8)) != QK_OK) return st;
+    int64_t gchunk = 0;
+    const int gchunks = lsm_gram_chunks(n, &gchunk);
+    const size_t gwin = (size_t)std::max(1, std::min(N - 1, lsm_gram_window(gchunks)));
+    if ((st = ctx->gpart.reserve(ctx, gwin * gchunks * LsmGram<5>::STRIDE * 8)) != QK_OK) return st;
+    if ((st = ctx->factor.reserve(ctx, (size_t)N * LSM_FACTOR_STRIDE * 8)) != QK_OK) return st;
     if ((st = ctx->psum.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pm2.reserve(ctx, n_tiles * 8)) != QK_OK) return st;
     if ((st = ctx->pitm.reserve(ctx, n_tiles * 4)) != QK_OK) return st;
-    if ((st = ctx->disc.reserve(ctx, 8)) != QK_OK) return st;
-    if ((st = ctx->scale.reserve(ctx, 8)) != QK_OK) return st;
     if ((st = ctx->stats.reserve(ctx, sizeof(qk_stats))) != QK_OK) return st;
     if ((st = ctx->mscratch.reserve(ctx, merge_chunks(n_tiles) * MERGE_MOM_BYTES)) != QK_OK) return st;
     hipStream_t s = ctx->stream;
-    const double one = 1.0, scale = c.price_scale;
-    QK_HIP(hipMemcpyAsync(ctx->disc.p, &one, 8, hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemcpyAsync(ctx->scale.p, &scale, 8, hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemsetAsync(ctx->coeffs.p, 0, (size_t)N * n_basis * 8, s));
+    // Gram matrices of every date (they depend only on the stored spots),
+    // factorised before the backward sweep.
+    LsmGramArgs<Real> ga{};
+    ga.store = (const Real *)ctx->store.p;
+    ga.gpart = (double *)ctx->gpart.p;
+    ga.factor = (double *)ctx->factor.p;
+    ga.n_paths = n;
+    ga.gchunk = gchunk;
+    ga.n_steps = N;
+    ga.right = right;
+    ga.gchunks = gchunks;
+    ga.K = (Real)K;
+    QK_HIP(launch_lsm_gram<Real>(ga, n_basis, weighted, s));
     PumpArgs<Real> a{};
     PumpReq<Real> &q = a.r[0];
     q.store = (const Real *)ctx->store.p;
     q.cf = (Real *)ctx->cf.p;
     q.bpart = (double *)ctx->bpart.p;
+    q.factor = (const double *)ctx->factor.p;
     q.coeffs = (double *)ctx->coeffs.p;
     q.psum = (double *)ctx->psum.p;
     q.pm2 = (double *)ctx->pm2.p;
@@ -964,6 +986,7 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
         ca.store = q.store;
         ca.cf = q.cf;
         ca.bpart = q.bpart;
+        ca.factor = q.factor;
         ca.coeffs = q.coeffs;
         ca.psum = q.psum;
         ca.pm2 = q.pm2;
@@ -982,12 +1005,12 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     } else {
         for (int t = N - 1; t >= 0; --t) {               // ForwardBackwardMC.hpp:219-242
             q.t = t;
-            QK_HIP(launch_pump<Real>(a, n_basis, weighted, lsm_group<Real>(), s));
+            QK_HIP(launch_pump<Real>(a, n_basis, weighted, s));
         }
     }
     QK_HIP(hipEventRecord(ctx->ev[3], s));
     QK_HIP(launch_merge((const double *)ctx->psum.p, (const double *)ctx->pm2.p, (const uint32_t *)ctx->pitm.p,
-                        n_tiles, n, 1, 1, (const double *)ctx->disc.p, (const double *)ctx->scale.p,
+                        n_tiles, n, 1, 1, (const double *)ctx->disc.p, (const double *)ctx->disc.p + 1,
                         ctx->mscratch.p, (qk_stats *)ctx->stats.p, s));
     QK_HIP(hipEventRecord(ctx->ev[2], s));
     QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, s));

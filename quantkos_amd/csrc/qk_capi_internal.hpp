@@ -43,6 +43,7 @@ struct qk_ctx {
     std::string err;
     qk_timing timing{};
     qk::DevBuf lanes, pays, psum, pm2, pitm, disc, scale, stats, greeks, store, cf, coeffs, bpart, counter, sums, mscratch;
+    qk::DevBuf gpart, factor; // Longstaff-Schwartz Gram pre-pass partials and per-date factor records
     qk::DevBuf payv, hist;   // per-path payoffs and selection histograms of qk_price_distribution
     qk::DevBuf tpose;        // [path][step] window of the path store (qk_copy_stored_paths)
     // Shape of the complete path set in `store` (store_paths = 0: none), left

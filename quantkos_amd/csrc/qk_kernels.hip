@@ -11,12 +11,18 @@
 //  k_merge      fixed-order Chan merge of tile moments + FP64 finalisation
 //               (OptionPricer.hpp:152-172, :204-249).
 //  k_greeks     FP64 central differences of lane prices (MCEngine.hpp:586-730).
-//  k_lsm_step   one Longstaff-Schwartz date for every listed request: FP64
-//               Cholesky solve of the date's normal equations, cash-flow update,
-//               Laguerre Gram / right-hand side of the previous date, and at
-//               date 0 the final discount and tile moments (LSMAmerican.hpp:
-//               310-341, :393-419, :433-473, :516-613).
+//  k_lsm_gram   pre-pass of a Longstaff-Schwartz request: every date's
+//  k_lsm_factor in-the-money count and Laguerre Gram matrix, summed in a fixed
+//               order and Cholesky-factorised in FP64 (LSMAmerican.hpp:433-473,
+//               :516-581).
+//  k_lsm_step   one Longstaff-Schwartz date for every listed request: solve of
+//               the date's normal equations with its factor, cash-flow update,
+//               right-hand side of the previous date, and at date 0 the final
+//               discount and tile moments (LSMAmerican.hpp:310-341, :393-419,
+//               :433-473, :484-502, :605-613).
 //  k_lsm_coop   the same dates of one request in one cooperative launch.
+//  k_lsm_solve  the request server's pump step (Gram and right-hand side
+//  k_lsm_paths  accumulated together, factorised per date by one block).
 //  k_select_hist  order statistics of per-path payoffs (distribution).
 #include <hip/hip_runtime.h>
 
@@ -375,23 +381,24 @@ __device__ __forceinline__ void lsm_basis(Real X, Real L[NB]) {
     }
 }
 
-// Cholesky solve with the reference's 1e-8 ridge, 1e-14 pivot limit and
-// mean-payoff fallback (LSMAmerican.hpp:516-581, :605-613).  s is laid out as
-// LsmSums<NB>: {count, upper triangle of M row by row, B}.
+// Cholesky factorisation of one date's Gram matrix with the reference's 1e-8
+// ridge, 1e-14 pivot limit and mean-payoff fallback (LSMAmerican.hpp:516-581,
+// :605-613).  s is laid out as LsmGram<NB> {count, upper triangle of M row by
+// row}; F receives the factor record (LsmFactor<NB>): count, fallback flag,
+// the strictly lower entries and the reciprocal diagonal.  Fallback dates are
+// those with fewer than NB in-the-money paths or a pivot below 1e-14.
 template <int NB>
-__device__ void lsm_solve(const double *s, double C[NB]) {
-    using S = LsmSums<NB>;
+__device__ void lsm_factor(const double *s, double F[LSM_FACTOR_STRIDE]) {
+    using S = LsmGram<NB>;
+    using FR = LsmFactor<NB>;
     const double count = s[0];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) C[k] = 0.0;
+    for (int k = 0; k < LSM_FACTOR_STRIDE; ++k) F[k] = 0.0;
+    F[0] = count;
     if (count < (double)NB) {
-        if (count > 0.0) C[0] = s[S::b(0)] / count;
+        F[1] = 1.0;
         return;
     }
-    // One thread runs this between two block barriers, so its latency is on
-    // the date's critical path: the divisions by the diagonal are products
-    // with its reciprocal (Math<double> sqrt and reciprocal, each within
-    // 0.5 ulp), which shortens the dependent chain.
     double Lm[NB][NB] = {};
     double inv[NB];
 #pragma unroll
@@ -405,8 +412,7 @@ __device__ void lsm_solve(const double *s, double C[NB]) {
             if (i == j) {
                 const double val = Mij - acc;
                 if (val < 1e-14) {
-                    C[0] = s[S::b(0)] / count;
-                    for (int k = 1; k < NB; ++k) C[k] = 0.0;
+                    F[1] = 1.0;
                     return;
                 }
                 Lm[i][i] = Math<double>::sqrt(val);
@@ -416,103 +422,207 @@ __device__ void lsm_solve(const double *s, double C[NB]) {
             }
         }
     }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+#pragma unroll
+        for (int j = 0; j < i; ++j) F[FR::l(i, j)] = Lm[i][j];
+        F[FR::inv(i)] = inv[i];
+    }
+}
+
+// Coefficients of one date from its factor record F and right-hand side B:
+// forward and back substitution with the factor, or the fallback C0 = B0 /
+// count (LSMAmerican.hpp:605-613; all zero without in-the-money paths).
+template <int NB>
+__device__ __forceinline__ void lsm_solve_factored(const double *F, const double *B, double C[NB]) {
+    using FR = LsmFactor<NB>;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) C[k] = 0.0;
+    if (F[1] != 0.0) {
+        if (F[0] > 0.0) C[0] = B[0] / F[0];
+        return;
+    }
     double y[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < i; ++k) acc += Lm[i][k] * y[k];
-        y[i] = (s[S::b(i)] - acc) * inv[i];
+        for (int k = 0; k < i; ++k) acc += F[FR::l(i, k)] * y[k];
+        y[i] = (B[i] - acc) * F[FR::inv(i)];
     }
 #pragma unroll
     for (int i = NB - 1; i >= 0; --i) {
         double acc = 0.0;
 #pragma unroll
-        for (int k = i + 1; k < NB; ++k) acc += Lm[k][i] * C[k];
-        C[i] = (y[i] - acc) * inv[i];
+        for (int k = i + 1; k < NB; ++k) acc += F[FR::l(k, i)] * C[k];
+        C[i] = (y[i] - acc) * F[FR::inv(i)];
     }
 }
 
-// Sums the block partials [nb][STRIDE] of one date's regression in a fixed
-// order and solves the NB x NB normal equations in FP64 (G threads; the
-// partials are read as one coalesced stream: STRIDE divides G, so thread i
-// always meets component i % STRIDE).  Every thread of the block must call it
-// (it holds block barriers); only `active` threads read, and thread 0 of an
-// active group leaves the coefficients in C.
-template <int NB, int G>
-__device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpart, int nb, double *red, bool active,
-                                                 double C[NB]) {
-    using S = LsmSums<NB>;
-    constexpr int NS = S::NS, ST = S::STRIDE;
-    const int lt = threadIdx.x & (G - 1);
-    constexpr int UR = 16;  // loads in flight per thread; partial sums combined in a fixed order
-    double acc[UR];
+// Wavefront sums of KP (4, 8, 16 or 32) components held by every lane, by a
+// reduce-scatter butterfly: at each level every lane keeps half of its values
+// and adds the partner's copy of them, so KP components cost KP - 1
+// exchanges (plus 6 - log2 KP full steps) instead of 6 per component.  The
+// partners differ in lane bit 5, 4 (permlane swaps, which need no select), 3,
+// 2 (row mirrors) and 1, 0 (quad permutations); those masks span all six
+// bits, so every sum covers the 64 lanes once.  Lane l returns component
+// l >> (6 - log2 KP) summed over the wavefront (the same bits in every lane
+// of its group).  v is clobbered.
+template <int KP>
+__device__ __forceinline__ double wave_scatter_sum(double (&v)[KP], int lid) {
+    static_assert(KP == 4 || KP == 8 || KP == 16 || KP == 32, "component count");
 #pragma unroll
-    for (int u = 0; u < UR; ++u) acc[u] = 0.0;
-    const int64_t total = active ? (int64_t)nb * ST : 0;
-    for (int64_t e0 = lt; e0 < total; e0 += G * UR) {
-#pragma unroll
-        for (int u = 0; u < UR; ++u) {
-            const int64_t e = e0 + (int64_t)u * G;
-            if (e < total) acc[u] += bpart[e];
-        }
+    for (int k = 0; k < KP / 2; ++k) {            // bit 5: lanes >= 32 keep k + KP/2
+        swap_halves_f64<false>(v[k], v[k + KP / 2]);
+        v[k] += v[k + KP / 2];
     }
-    double tot = 0.0;
 #pragma unroll
-    for (int u = 0; u < UR; ++u) tot += acc[u];
-    // Lanes l and l + ST of a wavefront hold the same component: two (ST = 16)
-    // or one (ST = 32) exchange steps leave each component's wavefront sum in
-    // lanes 0 .. ST-1; one barrier; then lane k of wavefront 0 adds component
-    // k over the wavefronts in order and thread 0 gathers the NS sums.
-    const int lid = lt & 63, wv = lt >> 6;
-    if constexpr (ST == 16) tot = add_across<true>(tot);
-    tot = add_across<false>(tot);
-    if (active && lid < ST) red[wv * ST + lid] = tot;
+    for (int k = 0; k < KP / 4; ++k) {            // bit 4
+        swap_halves_f64<true>(v[k], v[k + KP / 4]);
+        v[k] += v[k + KP / 4];
+    }
+    // bits 3, 2, 1: the lane with the bit set keeps the upper half and sends
+    // the lower one.
+    auto level = [&](auto ctrl, int h, int bit) {
+        const bool up = (lid & bit) != 0;
+#pragma unroll
+        for (int k = 0; k < h; ++k) {
+            const double keep = up ? v[k + h] : v[k];
+            const double send = up ? v[k] : v[k + h];
+            v[k] = keep + dpp_f64<decltype(ctrl)::value>(send);
+        }
+    };
+    if constexpr (KP >= 8) level(std::integral_constant<int, DPP_MIRROR>{}, KP / 8, 8);
+    if constexpr (KP >= 16) level(std::integral_constant<int, DPP_HALF_MIRROR>{}, KP / 16, 4);
+    if constexpr (KP >= 32) level(std::integral_constant<int, DPP_QP_XOR2>{}, 1, 2);
+    double sx = v[0];
+    if constexpr (KP < 8) sx += dpp_f64<DPP_MIRROR>(sx);
+    if constexpr (KP < 16) sx += dpp_f64<DPP_HALF_MIRROR>(sx);
+    if constexpr (KP < 32) sx += dpp_f64<DPP_QP_XOR2>(sx);
+    sx += dpp_f64<DPP_QP_XOR1>(sx);
+    return sx;
+}
+template <int KP>
+constexpr int scatter_shift() { return KP == 4 ? 4 : KP == 8 ? 3 : KP == 16 ? 2 : 1; }
+
+// Partial layout of the date step: the right-hand side only (GRAM = false,
+// the Gram matrix comes from the pre-pass) or Gram and right-hand side
+// together (GRAM = true, the request server's pump).
+template <int NB, bool GRAM>
+constexpr int lsm_part_stride() { return GRAM ? LsmSums<NB>::STRIDE : LsmRhs<NB>::STRIDE; }
+
+// Sums the block partials [nb][lsm_part_stride<NB, GRAM>()] of one date in a
+// fixed order (per thread, then across the wavefront, then the wavefronts in
+// order).  Every thread of the
+// block must call it (it holds one block barrier); every wavefront returns
+// the NV = lsm_part_count<NB, GRAM>() sums in sm, the same bits in all (each
+// wavefront sums the wavefront totals itself, so no second barrier is needed
+// to share them).
+template <int NB, bool GRAM>
+constexpr int lsm_part_count() { return GRAM ? LsmSums<NB>::NS : NB; }
+template <int NB, int G, bool GRAM>
+__device__ __forceinline__ void lsm_reduce_parts(const double *__restrict__ bpart, int nb, double *red,
+                                                 double sm[lsm_part_count<NB, GRAM>()]) {
+    constexpr int ST = lsm_part_stride<NB, GRAM>();
+    const int lt = threadIdx.x & (G - 1), lid = lt & 63, wv = lt >> 6;
+    if constexpr (ST >= 16) {
+        // Wide partials as one coalesced stream: thread i adds elements i,
+        // i + G, ... of [nb][ST] (ST divides G, so always component i % ST),
+        // then lanes l, l + ST, ... of the wavefront are added by one or two
+        // exchanges across lane bits 4 and 5.
+        constexpr int UR = 16;   // loads in flight per thread, combined in a fixed order
+        double acc[UR];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) acc[u] = 0.0;
+        const int64_t total = (int64_t)nb * ST;
+        for (int64_t e0 = lt; e0 < total; e0 += (int64_t)G * UR) {
+#pragma unroll
+            for (int u = 0; u < UR; ++u) {
+                const int64_t e = e0 + (int64_t)u * G;
+                if (e < total) acc[u] += bpart[e];
+            }
+        }
+        double tot = 0.0;
+#pragma unroll
+        for (int u = 0; u < UR; ++u) tot += acc[u];
+        if constexpr (ST == 16) tot = add_across<true>(tot);
+        tot = add_across<false>(tot);
+        if (lid < ST) red[wv * ST + lid] = tot;
+    } else {
+        // Narrow partials: thread i adds whole partials i, i + G, ..., then
+        // the wavefront reduce-scatter.
+        double v[ST];
+#pragma unroll
+        for (int k = 0; k < ST; ++k) v[k] = 0.0;
+        for (int p = lt; p < nb; p += G) {
+#pragma unroll
+            for (int k = 0; k < ST; ++k) v[k] += bpart[(int64_t)p * ST + k];
+        }
+        const double sx = wave_scatter_sum<ST>(v, lid);
+        constexpr int SH = scatter_shift<ST>();
+        if ((lid & ((1 << SH) - 1)) == 0) red[wv * ST + (lid >> SH)] = sx;
+    }
     __syncthreads();
-    if (active && wv == 0) {
-        double v = 0.0;
-        if (lid < ST)
-            for (int w = 0; w < G / 64; ++w) v += red[w * ST + lid];
-        double sm[NS];
-        const uint64_t vb = (uint64_t)__double_as_longlong(v);
+    double b = 0.0;
+    if (lid < ST)
+        for (int w = 0; w < G / 64; ++w) b += red[w * ST + lid];
+    const uint64_t bb = (uint64_t)__double_as_longlong(b);
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {   // lane k's component into scalar registers
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)vb, k);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(vb >> 32), k);
-            sm[k] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-        }
-        if (lid == 0) lsm_solve<NB>(sm, C);
+    for (int k = 0; k < lsm_part_count<NB, GRAM>(); ++k) {   // lane k's component into scalar registers
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bb, k);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bb >> 32), k);
+        sm[k] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
     }
+}
+
+// The right-hand-side partials of one date summed as above and solved with
+// the date's pre-computed factor record F; every thread returns the
+// coefficients in C, the same bits in all.
+template <int NB, int G>
+__device__ __forceinline__ void lsm_reduce_solve(const double *__restrict__ bpart, int nb,
+                                                 const double *__restrict__ F, double *red, double C[NB]) {
+    constexpr int NF = LsmFactor<NB>::NF;
+    double Fr[NF];   // uniform: loaded while the partials are in flight
+#pragma unroll
+    for (int k = 0; k < NF; ++k) Fr[k] = F[k];
+    double sm[NB];
+    lsm_reduce_parts<NB, G, false>(bpart, nb, red, sm);
+    lsm_solve_factored<NB>(Fr, sm, C);
 }
 
 // One Longstaff-Schwartz date t over the paths [p0, p1) by a group of G
 // threads (threadIdx.x & (G - 1) is the thread's place in the group): (1) updates
 // the cash flows at date t with the coefficients C of date t (initialises them
-// at the last date); (2) for t >= 1 accumulates the in-the-money Gram sums of
-// date t-1 in FP64 and stores the group's sums in part[0 .. STRIDE) (nothing
-// if part is null); (3) at date 0 applies the final one-step discount and
+// at the last date); (2) for t >= 1 accumulates the in-the-money right-hand
+// side sum L(S(t-1)) Y of date t-1 in FP64 (Y the discounted new cash flow;
+// the Gram matrix comes from the pre-pass) and stores the group's sums in
+// part[0 .. lsm_part_stride<NB, GRAM>()) (nothing if part is null); with GRAM
+// the in-the-money count and sum L L^T of date t-1 are accumulated too (the
+// request server's pump, which has no pre-pass); (3) at date 0 applies the
+// final one-step discount and
 // writes the per-tile moments of the cash flows.  cfp[i - p0] is the cash flow
 // of path i (device memory, or LDS when CF_LDS).  Holds a block barrier when t >= 1: every
 // thread of the block must call it for the same t.
-template <typename Real, int NB, bool W, bool CF_LDS, int G, typename Pre>
+template <typename Real, int NB, bool W, bool CF_LDS, int G, int U, bool GRAM, typename Pre>
 __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, Real *cfp, int64_t n, int64_t p0,
                                                int64_t p1, int t, int N, int right, Real K, Real disc, Real itm_eps,
                                                double C[NB], double *part, double *psum, double *pm2,
-                                               uint32_t *pitm, double (*wpart)[LsmSums<NB>::NS], Pre &&pre) {
-    using S = LsmSums<NB>;
-    constexpr int NS = S::NS;
+                                               uint32_t *pitm, double (*wpart)[lsm_part_stride<NB, GRAM>()],
+                                               Pre &&pre) {
+    constexpr int KP = lsm_part_stride<NB, GRAM>(), NV = lsm_part_count<NB, GRAM>();
     const int lt = threadIdx.x & (G - 1), lid = lt & 63, wv = lt >> 6;
-    double g[NS];
+    double g[NV];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) g[k] = 0.0;
-    // G-path rows per pass; a pass's loads are issued together, and the
-    // next pass's loads are issued before this pass is computed (they are
+    for (int k = 0; k < NV; ++k) g[k] = 0.0;
+    // U rows of G paths per pass; a pass's loads are issued together, and
+    // the next pass's loads are issued before this pass is computed (they are
     // other paths, so the cash-flow writes of this pass do not touch them).
+    // Thread lt meets paths lt, lt + G, lt + 2G, ... in that order whatever U
+    // is, so U changes no result.
     // Paths are addressed by 32-bit offsets o = i - p0 through buffer
     // resources over [p0, p1) of the date's row: rows past p1 read 0 without a
     // branch (they are then masked out).  Cash flows in LDS (CF_LDS) are read
     // with a bounds test instead.
-    constexpr int U = 8;
     constexpr uint32_t SZ = sizeof(Real);
     const uint32_t len = (uint32_t)(p1 - p0);
     const __amdgpu_buffer_rsrc_t r_s0 = buf_rsrc(store + (int64_t)t * n + p0, len * SZ);
@@ -583,15 +693,21 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
                         Real Lr[NB];
                         lsm_basis<Real, NB, W>(Math<Real>::div(S1, K), Lr);
                         const double Y = (double)(c * disc);
-                        double L[NB];
+                        if constexpr (GRAM) {
+                            using S = LsmSums<NB>;
+                            double L[NB];
 #pragma unroll
-                        for (int k = 0; k < NB; ++k) L[k] = (double)Lr[k];
-                        g[0] += 1.0;
+                            for (int k = 0; k < NB; ++k) L[k] = (double)Lr[k];
+                            g[0] += 1.0;
 #pragma unroll
-                        for (int a2 = 0; a2 < NB; ++a2) {
+                            for (int a2 = 0; a2 < NB; ++a2) {
 #pragma unroll
-                            for (int b2 = a2; b2 < NB; ++b2) g[S::m(a2, b2)] += L[a2] * L[b2];
-                            g[S::b(a2)] += L[a2] * Y;
+                                for (int b2 = a2; b2 < NB; ++b2) g[LsmGram<NB>::m(a2, b2)] += L[a2] * L[b2];
+                                g[S::b(a2)] += L[a2] * Y;
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < NB; ++k) g[k] += (double)Lr[k] * Y;
                         }
                     }
                 } else {
@@ -621,135 +737,100 @@ __device__ __forceinline__ void lsm_chunk_date(const Real *__restrict__ store, R
         }
     }
     if (t >= 1) {
-        // Wavefront sums of the NS components by a reduce-scatter butterfly
-        // over KP = 16 or 32 padded components: at each level every lane
-        // keeps half of its values and adds the partner's copy of them, so
-        // KP components cost KP - 1 exchanges (plus 6 - log2 KP full steps)
-        // instead of 6 per component.  The partners differ in lane bit 5, 4
-        // (permlane swaps, which need no select), 3, 2 (row mirrors) and 1, 0
-        // (quad permutations); those masks span all six bits, so every sum
-        // covers the 64 lanes once.  Lane l ends with component
-        // l >> (6 - log2 KP) summed over the wavefront.
-        constexpr int KP = NS <= 16 ? 16 : 32;
-        constexpr int LOGK = KP == 16 ? 4 : 5;
+        // Wavefront sums of the NV components padded to KP, then the group's
+        // wavefronts in order.
         double v[KP];
 #pragma unroll
-        for (int k = 0; k < KP; ++k) v[k] = k < NS ? g[k] : 0.0;
-#pragma unroll
-        for (int k = 0; k < KP / 2; ++k) {            // bit 5: lanes >= 32 keep k + KP/2
-            swap_halves_f64<false>(v[k], v[k + KP / 2]);
-            v[k] += v[k + KP / 2];
-        }
-#pragma unroll
-        for (int k = 0; k < KP / 4; ++k) {            // bit 4
-            swap_halves_f64<true>(v[k], v[k + KP / 4]);
-            v[k] += v[k + KP / 4];
-        }
-        // bits 3, 2 (and 1 for KP = 32): the lane with the bit set keeps the
-        // upper half and sends the lower one.
-        auto level = [&](auto ctrl, int h, int bit) {
-            const bool up = (lid & bit) != 0;
-#pragma unroll
-            for (int k = 0; k < h; ++k) {
-                const double keep = up ? v[k + h] : v[k];
-                const double send = up ? v[k] : v[k + h];
-                v[k] = keep + dpp_f64<decltype(ctrl)::value>(send);
-            }
-        };
-        level(std::integral_constant<int, DPP_MIRROR>{}, KP / 8, 8);
-        level(std::integral_constant<int, DPP_HALF_MIRROR>{}, KP / 16, 4);
-        double sx;
-        if constexpr (KP == 32) {
-            level(std::integral_constant<int, DPP_QP_XOR2>{}, 1, 2);
-            sx = v[0] + dpp_f64<DPP_QP_XOR1>(v[0]);
-        } else {
-            sx = v[0] + dpp_f64<DPP_QP_XOR2>(v[0]);
-            sx += dpp_f64<DPP_QP_XOR1>(sx);
-        }
-        const int comp = lid >> (6 - LOGK);
-        if ((lid & ((1 << (6 - LOGK)) - 1)) == 0 && comp < NS) wpart[wv][comp] = sx;
+        for (int k = 0; k < KP; ++k) v[k] = k < NV ? g[k] : 0.0;
+        const double sx = wave_scatter_sum<KP>(v, lid);
+        constexpr int SH = scatter_shift<KP>();
+        const int comp = lid >> SH;
+        if ((lid & ((1 << SH) - 1)) == 0) wpart[wv][comp] = sx;
         __syncthreads();
-        if (part != nullptr && lt < S::STRIDE) {          // the padding components are written as zero
+        if (part != nullptr && lt < KP) {
             double sx = 0.0;
-            if (lt < NS)
-                for (int w = 0; w < G / 64; ++w) sx += wpart[w][lt];
+            for (int w = 0; w < G / 64; ++w) sx += wpart[w][lt];
             part[lt] = sx;
         }
     }
 }
 
 // One Longstaff-Schwartz date for every request listed in the arguments, one
-// launch per date: every block of request q first sums the Gram partials of
-// date t (written by the previous launch into buffer t & 1) in block order and
-// solves the normal equations itself (block 0 also stores the coefficients),
-// then runs the date over its `chunk` of paths and writes its partial of date
-// t-1 into buffer (t-1) & 1.  The redundant solves read nb partials per block;
-// nb is kept at most PUMP_MAX_BLOCKS for that reason.
-template <typename Real, int NB, bool W, int G>
+// launch per date: every block of request q first sums the right-hand-side
+// partials of date t (written by the previous launch into buffer t & 1) in
+// block order and solves with the date's pre-computed factor itself (block 0
+// also stores the coefficients), then runs the date over its `chunk` of paths
+// and writes its partial of date t-1 into buffer (t-1) & 1.  The redundant
+// solves read nb partials per block; nb is kept at most PUMP_MAX_BLOCKS for
+// that reason.
+template <typename Real, int NB, bool W, int G, int U = 2>
 __global__ __launch_bounds__(G) void k_lsm_step(PumpArgs<Real> a) {
-    using S = LsmSums<NB>;
+    constexpr int ST = LsmRhs<NB>::STRIDE;
     int r = 0;
     while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
     const PumpReq<Real> &q = a.r[r];
     const int b = (int)blockIdx.x - a.block_start[r];
     const int t = q.t, N = q.n_steps;
-    __shared__ double red[G];
-    __shared__ double Cs[NB];
-    __shared__ double wpart[G / 64][S::NS];
+    __shared__ double red[G / 64 * ST];
+    __shared__ double wpart[G / 64][ST];
     auto pre = [&](double C[NB]) {
-        if (t < N - 1) {
-            double Cn[NB];
-            lsm_reduce_solve<NB, G>(q.bpart + (int64_t)(t & 1) * q.nb * S::STRIDE, q.nb, red, true, Cn);
-            if (threadIdx.x == 0)
-                for (int k = 0; k < NB; ++k) {
-                    Cs[k] = Cn[k];
-                    if (b == 0) q.coeffs[(int64_t)t * NB + k] = Cn[k];
-                }
-        }
-        __syncthreads();
+        double Cn[NB];
 #pragma unroll
-        for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? Cs[k] : 0.0;
+        for (int k = 0; k < NB; ++k) Cn[k] = 0.0;
+        if (t < N - 1) {
+            lsm_reduce_solve<NB, G>(q.bpart + (int64_t)(t & 1) * q.nb * ST, q.nb,
+                                    q.factor + (int64_t)t * LSM_FACTOR_STRIDE, red, Cn);
+        }
+        if (b == 0 && threadIdx.x == 0)   // the last date's row is zero (no regression there)
+            for (int k = 0; k < NB; ++k) q.coeffs[(int64_t)t * NB + k] = Cn[k];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) C[k] = Cn[k];
     };
     double C[NB];
     const int64_t p0 = (int64_t)b * q.chunk;
     const int64_t p1 = p0 + q.chunk < q.n_paths ? p0 + q.chunk : q.n_paths;
-    double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * S::STRIDE + (int64_t)b * S::STRIDE : nullptr;
-    lsm_chunk_date<Real, NB, W, false, G>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
+    double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * ST + (int64_t)b * ST : nullptr;
+    lsm_chunk_date<Real, NB, W, false, G, U, false>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
                                           part, q.psum, q.pm2, q.pitm, wpart, pre);
 }
 
 // The request server's Longstaff-Schwartz step in two launches, for requests
 // split into many blocks: k_lsm_solve has one block per request, which sums
-// the request's nb Gram partials of date t in block order, solves the normal
-// equations and stores the date's coefficients; k_lsm_paths then runs the
-// date over every block's chunk with those coefficients (read after the
-// kernel boundary) and writes its partial of date t-1.  The solve no longer
-// sits in every block, so a request can take thousands of blocks and the
-// path work reaches the bandwidth a fused step with 256 re-read partials per
-// block cannot (the server's pump steps share the GPU with batch launches and
-// are long enough that one more launch per step costs nothing measurable).
+// the request's nb Gram and right-hand-side partials of date t in block
+// order, factorises, solves and stores the date's coefficients; k_lsm_paths
+// then runs the date over every block's chunk with those coefficients (read
+// after the kernel boundary) and writes its Gram and right-hand-side partial
+// of date t-1 (the server has no pre-pass: a second read of every American
+// path store costs the stream more bandwidth than the Gram sums cost here).  The solve does
+// not sit in every block, so a request can take thousands of blocks (the
+// server's pump steps share the GPU with batch launches and are long enough
+// that one more launch per step costs nothing measurable).
 template <typename Real, int NB, int G>
 __global__ __launch_bounds__(G) void k_lsm_solve(PumpArgs<Real> a) {
-    using S = LsmSums<NB>;
+    constexpr int ST = lsm_part_stride<NB, true>();
     const PumpReq<Real> &q = a.r[blockIdx.x];
     const int t = q.t;
     if (t >= q.n_steps - 1) return;                   // uniform per block: the last date needs no solve
-    __shared__ double red[G];
-    double C[NB];
-    lsm_reduce_solve<NB, G>(q.bpart + (int64_t)(t & 1) * q.nb * S::STRIDE, q.nb, red, true, C);
+    __shared__ double red[G / 64 * ST];
+    double sm[LsmSums<NB>::NS];
+    lsm_reduce_parts<NB, G, true>(q.bpart + (int64_t)(t & 1) * q.nb * ST, q.nb, red, sm);
+    if (threadIdx.x >= 64) return;                    // one wavefront factorises and solves
+    double F[LSM_FACTOR_STRIDE], C[NB];
+    lsm_factor<NB>(sm, F);
+    lsm_solve_factored<NB>(F, sm + LsmSums<NB>::NG, C);
     if (threadIdx.x == 0)
         for (int k = 0; k < NB; ++k) q.coeffs[(int64_t)t * NB + k] = C[k];
 }
 
 template <typename Real, int NB, bool W, int G>
 __global__ __launch_bounds__(G) void k_lsm_paths(PumpArgs<Real> a) {
-    using S = LsmSums<NB>;
+    constexpr int ST = lsm_part_stride<NB, true>();
     int r = 0;
     while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
     const PumpReq<Real> &q = a.r[r];
     const int b = (int)blockIdx.x - a.block_start[r];
     const int t = q.t, N = q.n_steps;
-    __shared__ double wpart[G / 64][S::NS];
+    __shared__ double wpart[G / 64][ST];
     auto pre = [&](double C[NB]) {
 #pragma unroll
         for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? q.coeffs[(int64_t)t * NB + k] : 0.0;
@@ -757,9 +838,108 @@ __global__ __launch_bounds__(G) void k_lsm_paths(PumpArgs<Real> a) {
     double C[NB];
     const int64_t p0 = (int64_t)b * q.chunk;
     const int64_t p1 = p0 + q.chunk < q.n_paths ? p0 + q.chunk : q.n_paths;
-    double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * S::STRIDE + (int64_t)b * S::STRIDE : nullptr;
-    lsm_chunk_date<Real, NB, W, false, G>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
+    double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * ST + (int64_t)b * ST : nullptr;
+    lsm_chunk_date<Real, NB, W, false, G, 8, true>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
                                           part, q.psum, q.pm2, q.pitm, wpart, pre);
+}
+
+// Gram pre-pass, block (chunk c, date d0 + d) of 256 threads: the count and
+// the upper triangle of sum L L^T over the in-the-money paths of chunk c at
+// that date (the same test and basis as the date step: LSMAmerican.hpp:
+// 433-473), summed in FP64 in a fixed order (each thread's paths in order,
+// the wavefront butterfly, the four wavefronts in order) into
+// gpart[d][c][0 .. 16).
+template <typename Real, int NB, bool W>
+__global__ __launch_bounds__(256) void k_lsm_gram(LsmGramArgs<Real> a, int d0) {
+    using S = LsmGram<NB>;
+    constexpr int G = 256, U = 8, KP = S::STRIDE;
+    constexpr uint32_t SZ = sizeof(Real);
+    const int lt = (int)threadIdx.x, lid = lt & 63, wv = lt >> 6;
+    const int c = (int)(blockIdx.x % (unsigned)a.gchunks);
+    const int d = (int)(blockIdx.x / (unsigned)a.gchunks);
+    const int64_t p0 = (int64_t)c * a.gchunk;
+    const int64_t p1 = p0 + a.gchunk < a.n_paths ? p0 + a.gchunk : a.n_paths;
+    const uint32_t len = (uint32_t)(p1 - p0);
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.store + (int64_t)(d0 + d) * a.n_paths + p0, len * SZ);
+    double g[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) g[k] = 0.0;
+    for (uint32_t r0 = 0; r0 < len; r0 += (uint32_t)G * U) {
+        Real Sv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) Sv[u] = buf_load<Real>(rs, (r0 + (uint32_t)(u * G + lt)) * SZ);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (r0 + (uint32_t)(u * G + lt) < len && vanilla<Real>(a.right, Sv[u], a.K) > Real(0)) {
+                Real Lr[NB];
+                lsm_basis<Real, NB, W>(Math<Real>::div(Sv[u], a.K), Lr);
+                double L[NB];
+#pragma unroll
+                for (int k = 0; k < NB; ++k) L[k] = (double)Lr[k];
+                g[0] += 1.0;
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+#pragma unroll
+                    for (int j = i; j < NB; ++j) g[S::m(i, j)] += L[i] * L[j];
+                }
+            }
+        }
+    }
+    __shared__ double wpart[G / 64][KP];
+    const double sx = wave_scatter_sum<KP>(g, lid);
+    constexpr int SH = scatter_shift<KP>();
+    if ((lid & ((1 << SH) - 1)) == 0) wpart[wv][lid >> SH] = sx;
+    __syncthreads();
+    if (lt < KP) {
+        double v = 0.0;
+        for (int w = 0; w < G / 64; ++w) v += wpart[w][lt];
+        a.gpart[((int64_t)d * a.gchunks + c) * KP + lt] = v;
+    }
+}
+
+// One block of 256 threads per date d0 + blockIdx.x: sums the date's
+// pre-pass partials in a fixed order (thread (g, k) adds component k of
+// chunks g, g + 16, g + 32, ... in order, then component k's 16 group sums
+// in order) and writes the date's factor record.
+template <int NB>
+__global__ __launch_bounds__(256) void k_lsm_factor(const double *__restrict__ gpart, int gchunks,
+                                                    double *__restrict__ factor, int d0) {
+    using S = LsmGram<NB>;
+    constexpr int KS = S::STRIDE, NGRP = 256 / KS, UR = 4;
+    static_assert(KS == 16, "16 components per partial");
+    const int d = (int)blockIdx.x, lt = (int)threadIdx.x, k = lt % KS, grp = lt / KS;
+    const double *gp = gpart + (int64_t)d * gchunks * KS;
+    double v = 0.0;
+    for (int c0 = grp; c0 < gchunks; c0 += NGRP * UR) {
+        double x[UR];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+            const int c = c0 + u * NGRP;
+            x[u] = c < gchunks ? gp[(int64_t)c * KS + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UR; ++u) v += x[u];
+    }
+    __shared__ double red[NGRP][KS];
+    red[grp][k] = v;
+    __syncthreads();
+    if (lt >= 64) return;
+    double tot = 0.0;
+    if (lt < KS)
+#pragma unroll
+        for (int g2 = 0; g2 < NGRP; ++g2) tot += red[g2][lt];
+    const uint64_t vb = (uint64_t)__double_as_longlong(tot);
+    double sm[S::NG];
+#pragma unroll
+    for (int j = 0; j < S::NG; ++j) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)vb, j);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(vb >> 32), j);
+        sm[j] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    }
+    double F[LSM_FACTOR_STRIDE];
+    lsm_factor<NB>(sm, F);
+    if (lt == 0)
+        for (int j = 0; j < LsmFactor<NB>::NF; ++j) factor[(int64_t)(d0 + d) * LSM_FACTOR_STRIDE + j] = F[j];
 }
 
 // Synchronisation of the cooperative backward pass (sync words, agent scope):
@@ -832,12 +1012,12 @@ __device__ __forceinline__ bool coop_wait(unsigned int *sync, unsigned int k) {
 // between two buffers by date parity.
 template <typename Real, int NB, bool W>
 __global__ __launch_bounds__(lsm_group<Real>()) void k_lsm_coop(LsmCoopArgs<Real> a) {
-    using S = LsmSums<NB>;
+    constexpr int ST = LsmRhs<NB>::STRIDE;
     constexpr int G = lsm_group<Real>();
     extern __shared__ unsigned char lds_raw[];
     Real *cf_lds = reinterpret_cast<Real *>(lds_raw);
-    __shared__ double wpart[G / 64][S::NS];
-    __shared__ double red[G];
+    __shared__ double wpart[G / 64][ST];
+    __shared__ double red[G / 64 * ST];
     __shared__ double Cs[NB];
     double *cpub = reinterpret_cast<double *>(reinterpret_cast<unsigned char *>(a.sync) + SYNC_COEF_BYTES);
     const int vb = (int)blockIdx.x;
@@ -846,20 +1026,23 @@ __global__ __launch_bounds__(lsm_group<Real>()) void k_lsm_coop(LsmCoopArgs<Real
     const int64_t p1 = live ? (p0 + a.chunk < a.n_paths ? p0 + a.chunk : a.n_paths) : 0;
     Real *cfp = cf_lds;
     const int N = a.n_steps;
+    if (vb == 0 && threadIdx.x == 0)   // the last date has no regression
+        for (int k = 0; k < NB; ++k) a.coeffs[(int64_t)(N - 1) * NB + k] = 0.0;
     unsigned int k_bar = 0;
     for (int t = N - 1; t >= 0; --t) {
         double C[NB];
 #pragma unroll
         for (int k = 0; k < NB; ++k) C[k] = t < N - 1 ? Cs[k] : 0.0;
         const int par = (t - 1) & 1;
-        double *part = (t >= 1 && live) ? a.bpart + (int64_t)par * a.nb * S::STRIDE + (int64_t)vb * S::STRIDE : nullptr;
-        lsm_chunk_date<Real, NB, W, true, G>(a.store, cfp, a.n_paths, p0, p1, t, N, a.right, a.K, a.disc, a.itm_eps, C,
+        double *part = (t >= 1 && live) ? a.bpart + (int64_t)par * a.nb * ST + (int64_t)vb * ST : nullptr;
+        lsm_chunk_date<Real, NB, W, true, G, 8, false>(a.store, cfp, a.n_paths, p0, p1, t, N, a.right, a.K, a.disc, a.itm_eps, C,
                                              part, a.psum, a.pm2, a.pitm, wpart, [](double *) {});
         if (t >= 1) {
             ++k_bar;
             if (coop_arrive(a.sync, k_bar)) {
                 double Cn[NB];
-                lsm_reduce_solve<NB, G>(a.bpart + (int64_t)par * a.nb * S::STRIDE, a.nb, red, true, Cn);
+                lsm_reduce_solve<NB, G>(a.bpart + (int64_t)par * a.nb * ST, a.nb,
+                                        a.factor + (int64_t)(t - 1) * LSM_FACTOR_STRIDE, red, Cn);
                 if (threadIdx.x == 0) {
                     for (int k = 0; k < NB; ++k) {
                         Cs[k] = Cn[k];
@@ -1150,6 +1333,52 @@ template hipError_t launch_transpose_paths<float>(const float *, int64_t, int32_
 template hipError_t launch_transpose_paths<double>(const double *, int64_t, int32_t, int64_t, int64_t, double *,
                                                    hipStream_t);
 
+int lsm_gram_chunks(int64_t n_paths, int64_t *gchunk) {
+    // at least 8,192 paths (four passes of the block's 2,048-path rows) and at
+    // most 64 chunks per date, at most 2^28 paths (32-bit byte offsets)
+    int64_t c = std::max<int64_t>(8192, (n_paths + 63) / 64);
+    c = std::min<int64_t>((c + 2047) / 2048 * 2048, (int64_t)1 << 28);
+    *gchunk = c;
+    return (int)((n_paths + c - 1) / c);
+}
+
+int lsm_gram_window(int gchunks) { return std::max(1, 65536 / std::max(gchunks, 1)); }
+
+size_t lsm_prepass_bytes(int64_t n_paths, int32_t n_steps) {
+    int64_t gchunk = 0;
+    const int gchunks = lsm_gram_chunks(n_paths, &gchunk);
+    const int64_t dates = std::max<int64_t>(1, std::min<int64_t>(n_steps - 1, lsm_gram_window(gchunks)));
+    return (size_t)dates * gchunks * LsmGram<5>::STRIDE * 8 + (size_t)n_steps * LSM_FACTOR_STRIDE * 8;
+}
+
+template <typename Real, int NB, bool W>
+static void launch_gram_nb(const LsmGramArgs<Real> &a, hipStream_t s) {
+    const int window = lsm_gram_window(a.gchunks);
+    for (int d0 = 0; d0 < a.n_steps - 1; d0 += window) {
+        const int nd = std::min(window, a.n_steps - 1 - d0);
+        hipLaunchKernelGGL((k_lsm_gram<Real, NB, W>), dim3((unsigned)(a.gchunks * nd)), dim3(256), 0, s, a, d0);
+        hipLaunchKernelGGL((k_lsm_factor<NB>), dim3((unsigned)nd), dim3(256), 0, s, (const double *)a.gpart,
+                           a.gchunks, a.factor, d0);
+    }
+}
+
+template <typename Real>
+hipError_t launch_lsm_gram(const LsmGramArgs<Real> &a, int n_basis, bool weighted, hipStream_t s) {
+    if (a.n_steps < 2) return hipSuccess;
+    switch (n_basis * 2 + (weighted ? 1 : 0)) {
+    case 6: launch_gram_nb<Real, 3, false>(a, s); break;
+    case 7: launch_gram_nb<Real, 3, true>(a, s); break;
+    case 8: launch_gram_nb<Real, 4, false>(a, s); break;
+    case 9: launch_gram_nb<Real, 4, true>(a, s); break;
+    case 10: launch_gram_nb<Real, 5, false>(a, s); break;
+    case 11: launch_gram_nb<Real, 5, true>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_lsm_gram<float>(const LsmGramArgs<float> &, int, bool, hipStream_t);
+template hipError_t launch_lsm_gram<double>(const LsmGramArgs<double> &, int, bool, hipStream_t);
+
 int pump_blocks(int64_t n_paths, int group, int max_blocks, int64_t *chunk) {
     const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((n_paths + group - 1) / group, max_blocks));
     const int64_t c = ((n_paths + nb - 1) / nb + 255) / 256 * 256;
@@ -1158,11 +1387,9 @@ int pump_blocks(int64_t n_paths, int group, int max_blocks, int64_t *chunk) {
 }
 
 template <typename Real, int NB, bool W>
-static void launch_pump_nb(const PumpArgs<Real> &a, int group, hipStream_t s) {
-    constexpr int GB = lsm_group<Real>();
-    if (group == GB) hipLaunchKernelGGL((k_lsm_step<Real, NB, W, GB>), dim3(a.block_start[a.n_req]), dim3(GB), 0, s, a);
-    else hipLaunchKernelGGL((k_lsm_step<Real, NB, W, LSM_SHARED_GROUP>), dim3(a.block_start[a.n_req]),
-                            dim3(LSM_SHARED_GROUP), 0, s, a);
+static void launch_pump_nb(const PumpArgs<Real> &a, hipStream_t s) {
+    constexpr int G = lsm_group<Real>();
+    hipLaunchKernelGGL((k_lsm_step<Real, NB, W, G>), dim3(a.block_start[a.n_req]), dim3(G), 0, s, a);
 }
 
 template <typename Real, int NB, bool W>
@@ -1245,16 +1472,15 @@ int server_pump_blocks(int64_t n_paths, int64_t *chunk) {
 }
 
 template <typename Real>
-hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, int group, hipStream_t s) {
+hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s) {
     if (a.n_req <= 0) return hipSuccess;
-    if (group != lsm_group<Real>() && group != LSM_SHARED_GROUP) return hipErrorInvalidValue;
     switch (n_basis * 2 + (weighted ? 1 : 0)) {
-    case 6: launch_pump_nb<Real, 3, false>(a, group, s); break;
-    case 7: launch_pump_nb<Real, 3, true>(a, group, s); break;
-    case 8: launch_pump_nb<Real, 4, false>(a, group, s); break;
-    case 9: launch_pump_nb<Real, 4, true>(a, group, s); break;
-    case 10: launch_pump_nb<Real, 5, false>(a, group, s); break;
-    case 11: launch_pump_nb<Real, 5, true>(a, group, s); break;
+    case 6: launch_pump_nb<Real, 3, false>(a, s); break;
+    case 7: launch_pump_nb<Real, 3, true>(a, s); break;
+    case 8: launch_pump_nb<Real, 4, false>(a, s); break;
+    case 9: launch_pump_nb<Real, 4, true>(a, s); break;
+    case 10: launch_pump_nb<Real, 5, false>(a, s); break;
+    case 11: launch_pump_nb<Real, 5, true>(a, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1281,7 +1507,7 @@ template hipError_t launch_batch<double>(const ReqD<double> *, const ItemD *, in
                                          hipStream_t);
 template hipError_t launch_paths<float>(const PathArgs<float> &, int, int, bool, bool, int, hipStream_t);
 template hipError_t launch_paths<double>(const PathArgs<double> &, int, int, bool, bool, int, hipStream_t);
-template hipError_t launch_pump<float>(const PumpArgs<float> &, int, bool, int, hipStream_t);
-template hipError_t launch_pump<double>(const PumpArgs<double> &, int, bool, int, hipStream_t);
+template hipError_t launch_pump<float>(const PumpArgs<float> &, int, bool, hipStream_t);
+template hipError_t launch_pump<double>(const PumpArgs<double> &, int, bool, hipStream_t);
 
 }  // namespace qk

@@ -17,18 +17,71 @@ struct GreekMap {
     double h_S_real, h_vol, h_r, h_T;
 };
 
-// Layout of one date's regression sums for NB basis functions: count, the
-// upper triangle of the Gram matrix row by row, the right-hand side.
+// Longstaff-Schwartz regression of one date with NB basis functions.  The
+// left-hand side (the in-the-money count and sum L L^T) depends only on the
+// stored spots, so a pre-pass sums and factorises it for every date before the
+// backward sweep; the sweep's chain of dates carries only the right-hand side
+// sum L Y, which depends on the cash flows.
+// Pre-pass partial: count, then the upper triangle of sum L L^T row by row.
+template <int NB>
+struct LsmGram {
+    static constexpr int NM = NB * (NB + 1) / 2;
+    static constexpr int NG = 1 + NM;
+    static constexpr int STRIDE = 16;                   // doubles per partial (NG <= 16)
+    __host__ __device__ static constexpr int m(int i, int j) { return 1 + i * NB - i * (i - 1) / 2 + (j - i); }
+};
+// Right-hand side partial: NB components padded to a power of two.
+template <int NB>
+struct LsmRhs {
+    static constexpr int STRIDE = NB <= 4 ? 4 : 8;
+};
+// Factor record of one date (doubles): [0] in-the-money count, [1] 1 when the
+// date takes the mean-payoff fallback (LSMAmerican.hpp:536-545, :605-613), then
+// the strictly lower Cholesky entries of M + 1e-8 I row by row, then the NB
+// reciprocals of its diagonal.
+constexpr int LSM_FACTOR_STRIDE = 32;
+template <int NB>
+struct LsmFactor {
+    static constexpr int NL = NB * (NB - 1) / 2;
+    static constexpr int NF = 2 + NL + NB;              // entries in use
+    __host__ __device__ static constexpr int l(int i, int j) { return 2 + i * (i - 1) / 2 + j; }   // j < i
+    __host__ __device__ static constexpr int inv(int i) { return 2 + NL + i; }
+};
+// Partial of the request server's pump, which accumulates the Gram matrix
+// together with the right-hand side while it reads S(t-1) (no second read of
+// the path store; its solve block factorises): the pre-pass layout, then
+// sum L Y.
 template <int NB>
 struct LsmSums {
-    static constexpr int NM = NB * (NB + 1) / 2;
-    static constexpr int NS = 1 + NM + NB;
-    static constexpr int STRIDE = NS <= 16 ? 16 : 32;   // doubles per block partial; divides 256
-    __host__ __device__ static constexpr int m(int i, int j) { return 1 + i * NB - i * (i - 1) / 2 + (j - i); }
-    __host__ __device__ static constexpr int b(int k) { return 1 + NM + k; }
+    static constexpr int NG = LsmGram<NB>::NG;
+    static constexpr int NS = NG + NB;
+    static constexpr int STRIDE = NS <= 16 ? 16 : 32;   // doubles per partial
+    __host__ __device__ static constexpr int b(int k) { return NG + k; }
 };
 constexpr int LSM_MAX_BASIS = 5;
-constexpr int LSM_STRIDE = 32;          // block-partial allocation in doubles (>= every LsmSums<NB>::STRIDE)
+constexpr int LSM_STRIDE = 32;          // partial allocation in doubles (>= every LsmRhs / LsmSums STRIDE)
+
+// Gram pre-pass of one request: blocks of 256 threads over (chunk of paths,
+// date); dates are launched in windows of at most lsm_gram_window() dates,
+// whose partials [window][gchunks][LsmGram::STRIDE] one k_lsm_factor launch
+// sums (chunk order) and factorises into factor[date].
+template <typename Real>
+struct LsmGramArgs {
+    const Real *store;    // [n_steps][n_paths]
+    double *gpart;        // [window][gchunks][LsmGram<NB>::STRIDE]
+    double *factor;       // [n_steps][LSM_FACTOR_STRIDE]; dates 0 .. n_steps-2 are written
+    int64_t n_paths, gchunk;
+    int32_t n_steps, right, gchunks, pad;
+    Real K, pad2;
+};
+// Paths per pre-pass chunk and the number of chunks.
+int lsm_gram_chunks(int64_t n_paths, int64_t *gchunk);
+// Dates per pre-pass launch for the given chunk count.
+int lsm_gram_window(int gchunks);
+// Device bytes of the pre-pass: the window's partials and the factor records.
+size_t lsm_prepass_bytes(int64_t n_paths, int32_t n_steps);
+template <typename Real>
+hipError_t launch_lsm_gram(const LsmGramArgs<Real> &a, int n_basis, bool weighted, hipStream_t s);
 
 // One request of a Longstaff-Schwartz date step (k_lsm_step).
 constexpr int PUMP_MAX_REQ = 24;        // requests per pump launch (kernel-argument size)
@@ -37,7 +90,9 @@ template <typename Real>
 struct PumpReq {
     const Real *store;    // [n_steps][n_paths]
     Real *cf;             // [n_paths]
-    double *bpart;        // [2][nb][LsmSums<NB>::STRIDE] Gram partials, date d in buffer d & 1
+    double *bpart;        // [2][nb][STRIDE] partials, date d in buffer d & 1: LsmRhs (k_lsm_step) or
+                          // LsmSums (the server's k_lsm_solve / k_lsm_paths)
+    const double *factor; // [n_steps][LSM_FACTOR_STRIDE] from the Gram pre-pass (k_lsm_step only)
     double *coeffs;       // [n_steps][n_basis]
     double *psum, *pm2;   // tile moments of the final cash flows
     uint32_t *pitm;
@@ -81,7 +136,8 @@ template <typename Real>
 struct LsmCoopArgs {
     const Real *store;    // [n_steps][n_paths]
     Real *cf;             // [n_paths], final cash flows
-    double *bpart;        // [2][nb][STRIDE], by date parity
+    double *bpart;        // [2][nb][LsmRhs<NB>::STRIDE], by date parity
+    const double *factor; // [n_steps][LSM_FACTOR_STRIDE] from the Gram pre-pass
     double *coeffs;       // [n_steps][n_basis]
     double *psum, *pm2;
     uint32_t *pitm;
@@ -165,9 +221,8 @@ hipError_t launch_merge(const double *ps, const double *pm2, const uint32_t *pit
 hipError_t launch_merge_final(const void *chunks, int64_t n_chunks, int groups, int n_pay, const double *disc,
                               const double *scale, qk_stats *out, hipStream_t s);
 hipError_t launch_greeks(const qk_stats *st, int n_pay, const GreekMap &m, qk_greeks *out, hipStream_t s);
-// One k_lsm_step launch; group = threads per block, lsm_group<Real>() or
-// LSM_SHARED_GROUP (the blocks of every request must come from pump_blocks
-// with the same group).
+// One k_lsm_step launch of lsm_group<Real>() threads per block (the blocks of
+// every request must come from pump_blocks with that group).
 template <typename Real>
-hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, int group, hipStream_t s);
+hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s);
 }  // namespace qk

@@ -386,6 +386,7 @@ void fill_pump_req(PumpReq<Real> &q, const SReq *r) {
     q.store = (const Real *)r->dev(r->lay.store);
     q.cf = (Real *)r->dev(r->lay.cf);
     q.bpart = (double *)r->dev(r->lay.bpart);
+    q.factor = nullptr;   // the pump factorises the Gram sums it accumulates
     q.coeffs = (double *)r->dev(r->lay.coeffs);
     q.psum = (double *)r->dev(r->lay.psum);
     q.pm2 = (double *)r->dev(r->lay.pm2);
