@@ -282,10 +282,13 @@ qk_status qk_analytic_european(const qk_contract *c, int32_t right, double K, co
  * A request's result does not depend on the requests it was batched with. */
 typedef struct {
     size_t mem_budget_bytes;     /* device arena for admitted requests (0: 60% of free memory) */
-    int32_t max_inflight;        /* batch launches in flight, one stream each (0: 3)           */
+    int32_t max_inflight;        /* batch launches in flight, one stream each (0: 2)           */
     int32_t max_lsm_streams;     /* American requests in the backward sweep at once (0 or >96: 96, four chains of 24) */
     int64_t item_path_steps;     /* path-steps per work item (0: 131072)                      */
-    int32_t blocks_per_cu;       /* resident 256-thread blocks per CU of a batch launch (0: 8) */
+    int32_t blocks_per_cu;       /* resident 256-thread blocks per CU of a batch launch (0: 4) */
+    int32_t batch_items_per_wave; /* a batch takes requests until its work reaches this many work
+                                     items per resident wavefront (0: 32); a larger request still
+                                     forms a batch of its own */
 } qk_server_config;
 
 typedef struct {

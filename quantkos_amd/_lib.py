@@ -94,7 +94,7 @@ class CfQuadrature(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("mem_budget_bytes", C.c_size_t), ("max_inflight", C.c_int32),
                 ("max_lsm_streams", C.c_int32), ("item_path_steps", C.c_int64),
-                ("blocks_per_cu", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("batch_items_per_wave", C.c_int32)]
 
 
 class ScheduleEntry(C.Structure):

@@ -12,11 +12,15 @@ Knobs:
     More lanes per thread share one Philox block and one Box-Muller transform
     per step; fewer keep more waves resident and waste no padded lanes.
   * tile schedule of the request server: path-steps per work item (the tiles a
-    wavefront pulls at a time), resident blocks per CU of a batch launch, and
-    batch launches in flight.  Chosen by burst throughput first, then by p50
-    and p99 latency under Poisson arrivals among the candidates within 5 % of
-    the best throughput (larger items raise throughput little but make
-    batches coarser, which lengthens the typical request's wait).
+    wavefront pulls at a time), resident blocks per CU of a batch launch,
+    batch launches in flight, and the work cap of a batch (items per resident
+    wavefront).  Chosen by the burst throughput of the whole 4,096-request
+    stream (its American requests cycle the server's arena several times,
+    which a short stream does not show), then by p50 and p99 latency under
+    Poisson arrivals among the candidates within 5 % of the best throughput.
+    Fewer resident batch blocks leave registers for the Longstaff-Schwartz
+    pump's blocks beside a batch; smaller batches end more often, so
+    admission keeps the pump supplied.
 
 The reference has no counterpart: its launch shape is whatever Kokkos'
 RangePolicy picks (MSolver.hpp:195-232).
@@ -42,8 +46,8 @@ LANES_PER_THREAD = (1, 2, 4, 8)      # 8 in FP32 only
 # Lane counts tuned: the Greek stencils (12 lanes, 13 with the central theta
 # lane, 15 with every bump) and generic scenario fans.
 LANE_COUNTS = (2, 4, 8, 12, 13, 15, 16)
-SERVER_CANDIDATES = [dict(item_path_steps=ips, blocks_per_cu=bpc, max_inflight=mi)
-                     for ips in (65536, 131072, 524288) for bpc in (4, 8) for mi in (2, 3, 4)]
+SERVER_CANDIDATES = [dict(item_path_steps=65536, blocks_per_cu=bpc, max_inflight=mi, batch_items_per_wave=ipw)
+                     for bpc in (3, 4, 5, 6) for mi in (1, 2) for ipw in (16, 32)]
 # Burst-throughput band within which the server schedule is chosen by latency:
 # repeated burst measurements of one schedule differ by up to ~3 %.
 SERVER_RATE_TOLERANCE = 0.05
@@ -68,7 +72,7 @@ def validate(table: dict):
             raise ValueError("n_lanes must be in [1, 64]: %r" % (e,))
     srv = table.get("server")
     if srv is not None:
-        for k in ("item_path_steps", "blocks_per_cu", "max_inflight"):
+        for k in ("item_path_steps", "blocks_per_cu", "max_inflight", "batch_items_per_wave"):
             if int(srv.get(k, 0)) < 0:
                 raise ValueError("server.%s must be >= 0" % k)
 
@@ -152,7 +156,7 @@ def pick_server(rates: dict, latencies: dict, tolerance: float = SERVER_RATE_TOL
     return pick_best({k: math.sqrt(latencies[k][0] * latencies[k][1]) for k in timed})
 
 
-def tune_server(engine, n_requests=384, candidates=None, latency_requests=1024, repeats=SERVER_REPEATS,
+def tune_server(engine, n_requests=4096, candidates=None, latency_requests=2048, repeats=SERVER_REPEATS,
                 log=print):
     """Server schedule: burst throughput (path-steps/s, best of `repeats`
     runs) of the first `n_requests` requests of the config-4 stream (FP32)

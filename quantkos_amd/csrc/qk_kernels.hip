@@ -807,6 +807,7 @@ __global__ __launch_bounds__(G) void k_lsm_step(PumpArgs<Real> a) {
 // that one more launch per step costs nothing measurable).
 template <typename Real, int NB, int G>
 __global__ __launch_bounds__(G) void k_lsm_solve(PumpArgs<Real> a) {
+    __builtin_amdgcn_s_setprio(3);   // see k_lsm_paths
     constexpr int ST = lsm_part_stride<NB, true>();
     const PumpReq<Real> &q = a.r[blockIdx.x];
     const int t = q.t;
@@ -824,6 +825,13 @@ __global__ __launch_bounds__(G) void k_lsm_solve(PumpArgs<Real> a) {
 
 template <typename Real, int NB, bool W, int G>
 __global__ __launch_bounds__(G) void k_lsm_paths(PumpArgs<Real> a) {
+    // The pump's wavefronts share CUs with batch wavefronts that always have
+    // an arithmetic instruction ready; the SIMD issues the oldest ready
+    // wavefront first, so without a higher priority a pump step waits behind
+    // the batch (measured: tens of milliseconds per date beside a batch).
+    // Its memory-bound wavefronts issue few instructions, so the batch loses
+    // little.
+    __builtin_amdgcn_s_setprio(3);
     constexpr int ST = lsm_part_stride<NB, true>();
     int r = 0;
     while (r + 1 < a.n_req && (int)blockIdx.x >= a.block_start[r + 1]) ++r;
@@ -839,7 +847,7 @@ __global__ __launch_bounds__(G) void k_lsm_paths(PumpArgs<Real> a) {
     const int64_t p0 = (int64_t)b * q.chunk;
     const int64_t p1 = p0 + q.chunk < q.n_paths ? p0 + q.chunk : q.n_paths;
     double *part = t >= 1 ? q.bpart + (int64_t)((t - 1) & 1) * q.nb * ST + (int64_t)b * ST : nullptr;
-    lsm_chunk_date<Real, NB, W, false, G, 8, true>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
+    lsm_chunk_date<Real, NB, W, false, G, 2, true>(q.store, q.cf + p0, q.n_paths, p0, p1, t, N, q.right, q.K, q.disc, q.itm_eps, C,
                                           part, q.psum, q.pm2, q.pitm, wpart, pre);
 }
 

@@ -162,7 +162,7 @@ int pump_blocks(int64_t n_paths, int group, int max_blocks, int64_t *chunk);
 // reads at most 1,024 partials (128 KiB) per date.
 constexpr int64_t SERVER_PUMP_CHUNK = 2048;
 constexpr int64_t SERVER_PUMP_MAX_BLOCKS = 1024;
-constexpr int LSM_SOLVE_GROUP = 1024;   // threads of a k_lsm_solve block
+constexpr int LSM_SOLVE_GROUP = 256;    // threads of a k_lsm_solve block
 int server_pump_blocks(int64_t n_paths, int64_t *chunk);
 template <typename Real>
 hipError_t launch_pump_two_phase(const PumpArgs<Real> &a, int n_basis, bool weighted, hipStream_t s);

@@ -1,9 +1,10 @@
 // qk_server.hip -- continuous-batching request server (qk_server_* in include/qk.h).
 //
-// A scheduler thread admits pending requests in arrival order while their
-// device footprint fits a first-fit arena carved from one allocation, packs
-// them into a batch (work items = request x tile range), and launches it on a
-// free stream.  Up to max_inflight batches run at once, so requests arriving
+// A scheduler thread admits pending requests (American requests first, the
+// rest in arrival order) while their device footprint fits a first-fit arena
+// carved from one allocation, packs them into a batch (work items = request x
+// tile range; a batch's work is capped at batch_items_per_wave items per
+// resident wavefront), and launches it on a free stream.  Up to max_inflight batches run at once, so requests arriving
 // while others are on the GPU start as soon as a stream and memory free up.
 // American requests generate their path store inside a batch launch; once it
 // is ready they join the Longstaff-Schwartz pump, whose every launch advances
@@ -136,6 +137,11 @@ struct Slot {
     size_t hcap = 0;
     qk_stats *hstats = nullptr;
     size_t hstats_cap = 0;
+    // Outgrown staging buffers.  hipFree and hipHostFree wait for the whole
+    // device, which would stall the scheduler thread (and with it every pump
+    // chain) behind the batch running on the other slot, so they are released
+    // only when the server is destroyed.
+    std::vector<void *> retired_host, retired_dev;
 };
 
 }  // namespace
@@ -321,7 +327,7 @@ hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
     buf.resize(o_st + mreq.size() * sizeof(qk_stats) + 8);
     hipError_t e;
     if (slot.hcap < buf.size()) {
-        if (slot.hbuf) (void)hipHostFree(slot.hbuf);
+        if (slot.hbuf) slot.retired_host.push_back(slot.hbuf);
         slot.hcap = buf.size() * 2;
         if ((e = hipHostMalloc((void **)&slot.hbuf, slot.hcap, hipHostMallocDefault)) != hipSuccess) {
             slot.hbuf = nullptr;
@@ -330,10 +336,13 @@ hipError_t launch_slot(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
         }
     }
     if (slot.dbuf.cap < buf.size()) {
+        if (slot.dbuf.p) slot.retired_dev.push_back(slot.dbuf.p);
+        slot.dbuf.p = nullptr;
+        slot.dbuf.cap = 0;
         if (slot.dbuf.reserve(srv->ctx, buf.size() * 2) != QK_OK) return hipErrorOutOfMemory;
     }
     if (slot.hstats_cap < mreq.size() + 1) {
-        if (slot.hstats) (void)hipHostFree(slot.hstats);
+        if (slot.hstats) slot.retired_host.push_back(slot.hstats);
         slot.hstats_cap = (mreq.size() + 1) * 2;
         if ((e = hipHostMalloc((void **)&slot.hstats, slot.hstats_cap * sizeof(qk_stats), hipHostMallocDefault)) !=
             hipSuccess) {
@@ -593,9 +602,10 @@ bool schedule_once(qk_server *srv) {
     }
     if (pump(srv)) progress = true;
     if (progress) srv->cv_done.notify_all();
-    // Admission in arrival order while memory and a batch slot are free.  A
-    // request that does not fit the free memory stops admission of requests
-    // of its size, so large requests are not starved by smaller ones; small
+    // Admission while memory and a batch slot are free: American requests
+    // first (below), then the rest in arrival order.  A request that does
+    // not fit the free memory stops admission of requests of its size, so
+    // large requests are not starved by smaller ones; small
     // requests behind it (no path store, at most 1/64 of the arena: tile
     // moments only, released when their batch ends) may still go, so the
     // GPU keeps computing while the blocked request waits for memory that
@@ -610,8 +620,40 @@ bool schedule_once(qk_server *srv) {
     std::vector<SReq *> batch;
     bool blocked = false;
     int scanned = 0;
+    // A batch takes requests until its work reaches batch_items_per_wave
+    // items per resident wavefront: batches that end often let admission
+    // (and with it the American requests waiting for arena memory) proceed
+    // while the GPU computes, instead of one launch holding most of a burst.
+    const int64_t cap = (int64_t)srv->cfg.batch_items_per_wave * srv->n_cu * srv->cfg.blocks_per_cu *
+                        WAVES_PER_BLOCK * srv->cfg.item_path_steps;
+    int64_t batch_ps = 0;
+    auto admit = [&](SReq *r, size_t off) {
+        r->off = off;
+        r->base = srv->arena.base;
+        r->t_admit = clk::now();
+        r->admitted = true;
+        batch.push_back(r);
+        batch_ps += r->c.n_paths * (int64_t)r->c.n_steps;
+    };
+    // American requests first, in arrival order while they fit: their path
+    // stores hold arena memory from the forward launch to the end of the
+    // backward sweep, so they are the stream's bottleneck, and the batch
+    // computes the rest while their sweeps run (the pump's memory-bound
+    // date kernels overlap the batches' arithmetic).
+    for (auto it = srv->pending.begin(); it != srv->pending.end();) {
+        SReq *r = *it;
+        if (!r->american || r->lay.total > srv->arena.size) {
+            ++it;
+            continue;
+        }
+        const size_t off = srv->arena.alloc(r->lay.total);
+        if (off == SIZE_MAX) break;
+        admit(r, off);
+        it = srv->pending.erase(it);
+    }
     for (auto it = srv->pending.begin(); it != srv->pending.end() && scanned < ADMIT_SCAN;) {
         SReq *r = *it;
+        if (!batch.empty() && batch_ps >= cap) break;
         if (r->lay.total > srv->arena.size) {
             it = srv->pending.erase(it);
             r->err = "request footprint exceeds the server arena";
@@ -633,11 +675,7 @@ bool schedule_once(qk_server *srv) {
             ++it;
             continue;
         }
-        r->off = off;
-        r->base = srv->arena.base;
-        r->t_admit = clk::now();
-        r->admitted = true;
-        batch.push_back(r);
+        admit(r, off);
         it = srv->pending.erase(it);
     }
     if (batch.empty()) return progress;
@@ -695,11 +733,12 @@ extern "C" qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, 
         return fail(ctx, QK_EDEVICE, "hipMemGetInfo");
     }
     if (!s->cfg.mem_budget_bytes) s->cfg.mem_budget_bytes = (size_t)(0.6 * (double)free_b);
-    if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 3;
+    if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
     if (s->cfg.max_lsm_streams <= 0 || s->cfg.max_lsm_streams > N_PUMPS * PUMP_MAX_REQ)
         s->cfg.max_lsm_streams = N_PUMPS * PUMP_MAX_REQ;
     if (s->cfg.item_path_steps <= 0) s->cfg.item_path_steps = 131072;
-    if (s->cfg.blocks_per_cu <= 0) s->cfg.blocks_per_cu = 8;
+    if (s->cfg.blocks_per_cu <= 0) s->cfg.blocks_per_cu = 4;
+    if (s->cfg.batch_items_per_wave <= 0) s->cfg.batch_items_per_wave = 32;
     int n_cu = 0;
     if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && n_cu > 0)
         s->n_cu = n_cu;
@@ -744,6 +783,8 @@ extern "C" qk_status qk_server_destroy(qk_server *s) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.hbuf) (void)hipHostFree(sl.hbuf);
         if (sl.hstats) (void)hipHostFree(sl.hstats);
+        for (void *h : sl.retired_host) (void)hipHostFree(h);
+        for (void *d : sl.retired_dev) (void)hipFree(d);
         sl.dbuf.release();
     }
     for (PumpChain &pc : s->pumps) {

@@ -244,14 +244,17 @@ class Server:
     """A continuous-batching request server (qk_server_*) on an Engine's device."""
 
     def __init__(self, engine: Engine, mem_budget_bytes: int = 0, max_inflight: int = 0,
-                 max_lsm_streams: int = 0, item_path_steps: int = 0, blocks_per_cu: int = 0):
+                 max_lsm_streams: int = 0, item_path_steps: int = 0, blocks_per_cu: int = 0,
+                 batch_items_per_wave: int = 0):
         self.engine = engine
         self.L = engine.L
         tuned = (engine.tuning or {}).get("server") or {}
         max_inflight = max_inflight or int(tuned.get("max_inflight", 0))
         item_path_steps = item_path_steps or int(tuned.get("item_path_steps", 0))
         blocks_per_cu = blocks_per_cu or int(tuned.get("blocks_per_cu", 0))
-        cfg = ServerConfig(mem_budget_bytes, max_inflight, max_lsm_streams, item_path_steps, blocks_per_cu)
+        batch_items_per_wave = batch_items_per_wave or int(tuned.get("batch_items_per_wave", 0))
+        cfg = ServerConfig(mem_budget_bytes, max_inflight, max_lsm_streams, item_path_steps, blocks_per_cu,
+                           batch_items_per_wave)
         h = C.c_void_p()
         engine._check(self.L.qk_server_create(engine.h, C.byref(cfg), C.byref(h)))
         self.h = h
