@@ -364,6 +364,21 @@ static qk_status host_stage(qk_ctx *ctx, size_t bytes) {
     return QK_OK;
 }
 
+// Small device -> caller copies go through the pinned staging buffer: the
+// runtime carries out a copy into or out of pageable memory in a way that
+// waits for work on other streams (measured: a 0.5 ms American pricing
+// waited 2.3 s behind a request server's batches on another context), a
+// pinned one does not.  Queues the copy on ctx->stream, synchronises it and
+// copies out; the staging buffer must already hold `bytes` (host_stage before
+// any copy from it is queued: a regrowth frees the old buffer).
+static qk_status copy_out(qk_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes > ctx->hstage_cap) return fail(ctx, QK_EINVAL, "staging buffer too small");
+    QK_HIP(hipMemcpyAsync(ctx->hstage, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(dst, ctx->hstage, bytes);
+    return QK_OK;
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Byte layout of a lane call's constants block: lane constants, payoffs,
@@ -610,8 +625,9 @@ extern "C" qk_status qk_price_window(qk_ctx *ctx, const qk_contract *c, int64_t 
     if (st != QK_OK) return st;
     const int64_t n_tiles = (c->n_paths + TILE - 1) / TILE;
     const size_t bytes = (size_t)n_pay * (size_t)merge_chunks(n_tiles) * MERGE_MOM_BYTES;
-    QK_HIP(hipMemcpyAsync(chunks, ctx->mscratch.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    QK_HIP(hipStreamSynchronize(ctx->stream));
+    QK_HIP(hipStreamSynchronize(ctx->stream));           // nothing in flight reads the staging buffer
+    if ((st = host_stage(ctx, bytes)) != QK_OK) return st;
+    if ((st = copy_out(ctx, chunks, ctx->mscratch.p, bytes)) != QK_OK) return st;
     return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps, false);
 }
 
@@ -627,15 +643,19 @@ extern "C" qk_status qk_merge_windows(qk_ctx *ctx, const qk_contract *c, const q
     QK_HIP(hipSetDevice(ctx->device));
     // The discount and price scales as the lane call computes them.
     const ConstLayout lay = c->precision == QK_FP32 ? const_layout<float>(1, n_pay) : const_layout<double>(1, n_pay);
-    if ((st = host_stage(ctx, std::max(lay.bytes, sizeof(qk_stats) * n_pay))) != QK_OK) return st;
+    // constants at 0, the chunks after them, both from the pinned staging buffer
+    const size_t o_chunks = (lay.bytes + 255) / 256 * 256;
+    QK_HIP(hipEventSynchronize(ctx->ev[0]));
+    if ((st = host_stage(ctx, std::max(o_chunks + cbytes, sizeof(qk_stats) * n_pay))) != QK_OK) return st;
     if ((st = ctx->lanes.reserve(ctx, lay.bytes)) != QK_OK) return st;
     if ((st = ctx->mscratch.reserve(ctx, cbytes)) != QK_OK) return st;
     if ((st = ctx->stats.reserve(ctx, sizeof(qk_stats) * n_pay)) != QK_OK) return st;
-    QK_HIP(hipEventSynchronize(ctx->ev[0]));
     if (c->precision == QK_FP32) pack_consts<float>(c, 1, pays, n_pay, lay, (char *)ctx->hstage);
     else pack_consts<double>(c, 1, pays, n_pay, lay, (char *)ctx->hstage);
+    std::memcpy((char *)ctx->hstage + o_chunks, chunks, cbytes);
     QK_HIP(hipMemcpyAsync(ctx->lanes.p, ctx->hstage, lay.bytes, hipMemcpyHostToDevice, ctx->stream));
-    QK_HIP(hipMemcpyAsync(ctx->mscratch.p, chunks, cbytes, hipMemcpyHostToDevice, ctx->stream));
+    QK_HIP(hipMemcpyAsync(ctx->mscratch.p, (char *)ctx->hstage + o_chunks, cbytes, hipMemcpyHostToDevice,
+                          ctx->stream));
     QK_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     const char *cb = (const char *)ctx->lanes.p;
     QK_HIP(launch_merge_final(ctx->mscratch.p, n_chunks, n_pay, n_pay, (const double *)(cb + lay.o_disc),
@@ -862,8 +882,7 @@ extern "C" qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk
     if ((st = ctx->greeks.reserve(ctx, sizeof(qk_greeks) * n_pay)) != QK_OK) return st;
     QK_HIP(launch_greeks((const qk_stats *)ctx->stats.p, n_pay, m, (qk_greeks *)ctx->greeks.p, ctx->stream));
     QK_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    QK_HIP(hipMemcpyAsync(out, ctx->greeks.p, sizeof(qk_greeks) * n_pay, hipMemcpyDeviceToHost, ctx->stream));
-    QK_HIP(hipStreamSynchronize(ctx->stream));
+    if ((st = copy_out(ctx, out, ctx->greeks.p, sizeof(qk_greeks) * n_pay)) != QK_OK) return st;
     return finish_timing(ctx, c->n_paths * (int64_t)c->n_steps * (int64_t)L.size(), false);
 }
 
@@ -872,6 +891,10 @@ extern "C" qk_status qk_price_greeks(qk_ctx *ctx, const qk_contract *c, const qk
 // ---------------------------------------------------------------------------
 namespace qk {
 
+// Staging-buffer slots of the path-store calls: generate_store's lane
+// constants at 0, run_american's merge discount and scale after them.
+constexpr size_t STAGE_LANE_BYTES = 1024, STAGE_AUX = STAGE_LANE_BYTES;
+
 template <typename Real>
 static qk_status generate_store(qk_ctx *ctx, const qk_contract &c, int64_t path0 = 0) {
     const LaneC<Real> hl = make_lane<Real>(c);
@@ -879,7 +902,11 @@ static qk_status generate_store(qk_ctx *ctx, const qk_contract &c, int64_t path0
     if ((st = ctx->lanes.reserve(ctx, sizeof(LaneC<Real>))) != QK_OK) return st;
     ctx->store_paths = 0;   // the store holds no complete path set until the launch is queued
     if ((st = ctx->store.reserve(ctx, (size_t)c.n_paths * c.n_steps * sizeof(Real))) != QK_OK) return st;
-    QK_HIP(hipMemcpyAsync(ctx->lanes.p, &hl, sizeof(hl), hipMemcpyHostToDevice, ctx->stream));
+    // staging buffer: the lane at 0 (callers keep 0 .. STAGE_LANE_BYTES free of copies in flight)
+    static_assert(sizeof(hl) <= STAGE_LANE_BYTES, "lane constants exceed their staging slot");
+    if ((st = host_stage(ctx, STAGE_LANE_BYTES)) != QK_OK) return st;
+    std::memcpy(ctx->hstage, &hl, sizeof(hl));
+    QK_HIP(hipMemcpyAsync(ctx->lanes.p, ctx->hstage, sizeof(hl), hipMemcpyHostToDevice, ctx->stream));
     PathArgs<Real> a{};
     a.lanes = (const LaneC<Real> *)ctx->lanes.p;
     a.pays = nullptr;
@@ -907,11 +934,17 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
     // The merge's discount (1: no e^{-rT}, OptionPricer.hpp:211-216) and price
     // scale, in one copy from the pinned staging buffer ahead of the paths.
     qk_status st;
+    const int N_ = c.n_steps;
+    // Staging for the whole call, reserved before any copy from it is queued:
+    // lane constants and discount/scale going in, results and coefficients
+    // coming back (after the kernels, so they reuse the slots).
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    if ((st = host_stage(ctx, std::max<size_t>(STAGE_AUX + 16, 256 + (size_t)N_ * n_basis * 8))) != QK_OK) return st;
     if ((st = ctx->disc.reserve(ctx, 16)) != QK_OK) return st;
-    if ((st = host_stage(ctx, 16)) != QK_OK) return st;
-    static_cast<double *>(ctx->hstage)[0] = 1.0;
-    static_cast<double *>(ctx->hstage)[1] = c.price_scale;
-    QK_HIP(hipMemcpyAsync(ctx->disc.p, ctx->hstage, 16, hipMemcpyHostToDevice, ctx->stream));
+    double *aux = (double *)((char *)ctx->hstage + STAGE_AUX);
+    aux[0] = 1.0;
+    aux[1] = c.price_scale;
+    QK_HIP(hipMemcpyAsync(ctx->disc.p, aux, 16, hipMemcpyHostToDevice, ctx->stream));
     if ((st = generate_store<Real>(ctx, c)) != QK_OK) return st;
     const int64_t n = c.n_paths;
     const int N = c.n_steps;
@@ -1013,13 +1046,19 @@ static qk_status run_american(qk_ctx *ctx, const qk_contract &c, int right, doub
                         n_tiles, n, 1, 1, (const double *)ctx->disc.p, (const double *)ctx->disc.p + 1,
                         ctx->mscratch.p, (qk_stats *)ctx->stats.p, s));
     QK_HIP(hipEventRecord(ctx->ev[2], s));
-    QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, s));
+    // results through the staging buffer: stats at 0, the error word at 128,
+    // the coefficients at 256
+    char *hs = (char *)ctx->hstage;
+    QK_HIP(hipMemcpyAsync(hs, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, s));
     if (coeffs_out)
-        QK_HIP(hipMemcpyAsync(coeffs_out, ctx->coeffs.p, (size_t)N * n_basis * 8, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipMemcpyAsync(hs + 256, ctx->coeffs.p, (size_t)N * n_basis * 8, hipMemcpyDeviceToHost, s));
     unsigned int sync_err = 0;
-    if (coop) QK_HIP(hipMemcpyAsync(&sync_err, (char *)ctx->counter.p + 4 * LSM_COOP_SYNC_ERR_WORD, 4,
+    if (coop) QK_HIP(hipMemcpyAsync(hs + 128, (char *)ctx->counter.p + 4 * LSM_COOP_SYNC_ERR_WORD, 4,
                                     hipMemcpyDeviceToHost, s));
     QK_HIP(hipStreamSynchronize(s));
+    std::memcpy(out, hs, sizeof(qk_stats));
+    if (coeffs_out) std::memcpy(coeffs_out, hs + 256, (size_t)N * n_basis * 8);
+    if (coop) std::memcpy(&sync_err, hs + 128, 4);
     if (sync_err) return fail(ctx, QK_EDEVICE, "Longstaff-Schwartz grid barrier timed out");
     // Immediate exercise check (LSMAmerican.hpp:409-418): if today's intrinsic
     // beats the mean discounted cash flow, every path exercises at t0.
@@ -1088,6 +1127,8 @@ static qk_status select_ranks(qk_ctx *ctx, const Real *vals, int64_t n, const in
     constexpr int BITS = 8 * (int)sizeof(Real);
     qk_status st;
     if ((st = ctx->hist.reserve(ctx, SELECT_MAX_GROUPS * 256 * 4)) != QK_OK) return st;
+    QK_HIP(hipStreamSynchronize(ctx->stream));
+    if ((st = host_stage(ctx, SELECT_MAX_GROUPS * 256 * 4)) != QK_OK) return st;
     std::vector<uint64_t> prefix(n_ranks, 0);
     std::vector<int64_t> rest(ranks, ranks + n_ranks);
     std::vector<uint32_t> h(SELECT_MAX_GROUPS * 256);
@@ -1107,8 +1148,7 @@ static qk_status select_ranks(qk_ctx *ctx, const Real *vals, int64_t n, const in
         QK_HIP(hipMemsetAsync(ctx->hist.p, 0, (size_t)a.n_groups * 256 * 4, s));
         a.shift = shift;
         QK_HIP(launch_select_hist<Real>(vals, n, a, (uint32_t *)ctx->hist.p, blocks, s));
-        QK_HIP(hipMemcpyAsync(h.data(), ctx->hist.p, (size_t)a.n_groups * 256 * 4, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipStreamSynchronize(s));
+        if ((st = copy_out(ctx, h.data(), ctx->hist.p, (size_t)a.n_groups * 256 * 4)) != QK_OK) return st;
         for (int r = 0; r < n_ranks; ++r) {
             const uint32_t *hg = h.data() + group[r] * 256;
             int d = 0;
@@ -1136,8 +1176,7 @@ static qk_status run_distribution(qk_ctx *ctx, const qk_contract &c, const qk_pa
     } else {
         if ((st = ctx->payv.reserve(ctx, (size_t)n * sizeof(Real))) != QK_OK) return st;
         if ((st = run_lanes<Real>(ctx, &c, 1, &p, 1, (Real *)ctx->payv.p)) != QK_OK) return st;
-        QK_HIP(hipMemcpyAsync(out, ctx->stats.p, sizeof(qk_stats), hipMemcpyDeviceToHost, ctx->stream));
-        QK_HIP(hipStreamSynchronize(ctx->stream));
+        if ((st = copy_out(ctx, out, ctx->stats.p, sizeof(qk_stats))) != QK_OK) return st;
         if ((st = finish_timing(ctx, n * (int64_t)c.n_steps, false)) != QK_OK) return st;
         vals = (const Real *)ctx->payv.p;
         factor = std::exp(-c.r * c.T) * (p.type == QK_BINARY_CASH ? 1.0 : c.price_scale);
