@@ -500,7 +500,7 @@ int qko_price_american(const qko_model *m, int64_t n_paths, int32_t right, doubl
 int qko_price_american_ex(const qko_model *m, int64_t n_paths, int32_t right, double K, int32_t nb,
                           int32_t weighted, double price_scale, qko_stats *out, double *coeffs_out,
                           double *cf_out) {
-    if (!m || !out || n_paths < 1 || m->n_steps < 2 || m->n_steps > (1 << 30) || nb < 3 || nb > QKO_MAX_BASIS) return -1;
+    if (!m || !out || n_paths < 1 || m->n_steps < 1 || m->n_steps > (1 << 30) || nb < 3 || nb > QKO_MAX_BASIS) return -1;
     const int32_t N = m->n_steps;
     double *S = malloc(sizeof(double) * n_paths * (size_t)N);
     double *cf = malloc(sizeof(double) * n_paths);

@@ -247,13 +247,15 @@ def test_american_matches_oracle(engine, oracle, precision, model, scheme):
 
 
 @pytest.mark.parametrize("precision,n_paths,n_steps", [(qk.FP64, 1, 6), (qk.FP64, 1025, 6), (qk.FP32, 4097, 6),
-                                                       (qk.FP64, 1_500_000, 12), (qk.FP32, 2_600_000, 8)])
+                                                       (qk.FP64, 1_500_000, 12), (qk.FP32, 2_600_000, 8),
+                                                       (qk.FP64, 4096, 1), (qk.FP32, 4096, 2), (qk.FP64, 70_000, 3)])
 def test_american_chunk_edges_match_oracle(engine, oracle, precision, n_paths, n_steps):
     """Sizes at the edges of the date kernel's partition: a single path, a
     partial last row, and (1.5M FP64 / 2.6M FP32) chunks of more than one pass
-    of 8 rows per thread, whose second pass is loaded while the first is
-    computed, with a short last block.  Tolerances as in
-    test_american_matches_oracle."""
+    of two rows per thread, whose second pass is loaded while the first is
+    computed, with a short last block; one exercise date (no regression, no
+    Gram pre-pass) and two (one regression); more than one pre-pass chunk
+    (70,000 paths).  Tolerances as in test_american_matches_oracle."""
     c = base_contract(qk.HESTON, qk.EULER, precision, n_paths=n_paths, n_steps=n_steps, price_scale=100.0)
     g = engine.price_american(c, qk.PUT, 1.1)
     o = oracle.price_american(omodel(oracle, c), c.n_paths, qk.PUT, 1.1, 100.0)
@@ -264,6 +266,25 @@ def test_american_chunk_edges_match_oracle(engine, oracle, precision, n_paths, n
         assert abs(g.price - o.price) < 0.02 * o.se, (g.price, o.price, o.se)
     else:
         assert abs(g.price - o.price) < 0.5 * o.se, (g.price, o.price, o.se)
+
+
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+def test_american_deep_out_of_the_money_fallback(engine, oracle, precision):
+    """A put struck far below the spot: at the early dates fewer than four
+    paths are in the money, so the regression takes the mean-payoff fallback
+    (LSMAmerican.hpp:536-545, :605-613), at later dates it solves.  The
+    coefficients match the oracle's (fallback dates exactly in FP64)."""
+    c = base_contract(qk.HESTON, qk.EULER, precision, n_paths=20_000, n_steps=24, price_scale=100.0)
+    g, gc = engine.price_american(c, qk.PUT, 0.55, want_coeffs=True)
+    o = oracle.price_american(omodel(oracle, c), c.n_paths, qk.PUT, 0.55, 100.0)
+    assert g.n == o.n
+    if precision == qk.FP64:
+        assert abs(g.price - o.price) <= max(0.02 * o.se, 1e-9), (g.price, o.price, o.se)
+    else:
+        assert abs(g.price - o.price) <= max(0.5 * o.se, 1e-6), (g.price, o.price, o.se)
+    # early dates: at most one nonzero coefficient (the fallback mean)
+    fallback = [t for t in range(c.n_steps - 1) if all(x == 0.0 for x in gc[t][1:])]
+    assert fallback and fallback[0] == 0
 
 
 IKONEN_TOIVANEN = {8.0: 2.0, 9.0: 1.107641, 10.0: 0.520030, 11.0: 0.213668, 12.0: 0.082036}
