@@ -289,6 +289,9 @@ typedef struct {
     int32_t batch_items_per_wave; /* a batch takes requests until its work reaches this many work
                                      items per resident wavefront (0: 32); a larger request still
                                      forms a batch of its own */
+    int32_t overlap_min_backlog; /* a batch is launched beside one already in flight only while
+                                    the pending requests hold at least this many batches' work
+                                    (the work cap above); 0: whenever a stream is free */
 } qk_server_config;
 
 typedef struct {

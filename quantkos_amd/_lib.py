@@ -11,6 +11,17 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libquantkos.so")
 
+# The request server runs up to two batch streams and four Longstaff-Schwartz
+# pump streams beside the ctx stream.  HIP maps streams onto a per-process pool
+# of hardware queues (4 by default); streams beyond the pool share a queue, and
+# a pump stream that shares a batch stream's queue waits behind every batch
+# kernel (measured on the 4,096-request stream: that chain's dates ~10x slower,
+# American p99 latency 224-344 ms instead of ~75 ms).  Eight queues give every
+# stream its own.  HIP reads the setting when it initialises, so it takes effect
+# when this package is imported before anything else in the process uses the
+# GPU; a value set by the caller is kept.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 QK_OK, QK_EINVAL, QK_ENOMEM, QK_EDEVICE, QK_EAGAIN, QK_ENOTFOUND, QK_ENOTREADY = 0, -1, -2, -3, -4, -5, -6
 HESTON, BATES = 0, 1
 EULER, MILSTEIN, QE = 0, 1, 2
@@ -94,7 +105,8 @@ class CfQuadrature(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("mem_budget_bytes", C.c_size_t), ("max_inflight", C.c_int32),
                 ("max_lsm_streams", C.c_int32), ("item_path_steps", C.c_int64),
-                ("blocks_per_cu", C.c_int32), ("batch_items_per_wave", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("batch_items_per_wave", C.c_int32),
+                ("overlap_min_backlog", C.c_int32)]
 
 
 class ScheduleEntry(C.Structure):

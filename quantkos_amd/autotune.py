@@ -46,8 +46,12 @@ LANES_PER_THREAD = (1, 2, 4, 8)      # 8 in FP32 only
 # Lane counts tuned: the Greek stencils (12 lanes, 13 with the central theta
 # lane, 15 with every bump) and generic scenario fans.
 LANE_COUNTS = (2, 4, 8, 12, 13, 15, 16)
-SERVER_CANDIDATES = [dict(item_path_steps=65536, blocks_per_cu=bpc, max_inflight=mi, batch_items_per_wave=ipw)
-                     for bpc in (3, 4, 5, 6) for mi in (1, 2) for ipw in (16, 32)]
+# Two batches in flight fill each other's tails but share the GPU; with a
+# backlog gate of 2 the second one runs only while the queue holds two
+# batches' work (bursts), so light traffic keeps one batch at a time.
+SERVER_CANDIDATES = [dict(item_path_steps=65536, blocks_per_cu=bpc, max_inflight=mi, batch_items_per_wave=ipw,
+                          overlap_min_backlog=gate)
+                     for bpc in (3, 4, 5) for mi, gate in ((1, 0), (2, 0), (2, 2)) for ipw in (16, 32)]
 # Burst-throughput band within which the server schedule is chosen by latency:
 # repeated burst measurements of one schedule differ by up to ~3 %.
 SERVER_RATE_TOLERANCE = 0.05
@@ -72,7 +76,8 @@ def validate(table: dict):
             raise ValueError("n_lanes must be in [1, 64]: %r" % (e,))
     srv = table.get("server")
     if srv is not None:
-        for k in ("item_path_steps", "blocks_per_cu", "max_inflight", "batch_items_per_wave"):
+        for k in ("item_path_steps", "blocks_per_cu", "max_inflight", "batch_items_per_wave",
+                  "overlap_min_backlog"):
             if int(srv.get(k, 0)) < 0:
                 raise ValueError("server.%s must be >= 0" % k)
 

@@ -4,8 +4,10 @@
 // rest in arrival order) while their device footprint fits a first-fit arena
 // carved from one allocation, packs them into a batch (work items = request x
 // tile range; a batch's work is capped at batch_items_per_wave items per
-// resident wavefront), and launches it on a free stream.  Up to max_inflight batches run at once, so requests arriving
-// while others are on the GPU start as soon as a stream and memory free up.
+// resident wavefront), and launches it on a free stream.  Up to max_inflight
+// batches run at once (beyond the first only while the queue holds
+// overlap_min_backlog batches' work, if set), so requests arriving while
+// others are on the GPU start as soon as a stream and memory free up.
 // American requests generate their path store inside a batch launch; once it
 // is ready they join the Longstaff-Schwartz pump, whose every launch advances
 // all joined requests by one exercise date.  Completion is observed by polling
@@ -164,6 +166,7 @@ struct qk_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<SReq *> pending;
+    int64_t pending_ps = 0;                    // path-steps of the pending requests
     std::unordered_map<uint64_t, SReq *> reqs;
     uint64_t next_ticket = 1;
     bool stop = false;
@@ -611,22 +614,34 @@ bool schedule_once(qk_server *srv) {
     // GPU keeps computing while the blocked request waits for memory that
     // the Longstaff-Schwartz pump is freeing.
     Slot *free_slot = nullptr;
-    for (Slot &slot : srv->slots)
-        if (!slot.busy) {
-            free_slot = &slot;
-            break;
-        }
+    int n_busy_slots = 0;
+    for (Slot &slot : srv->slots) {
+        if (slot.busy) ++n_busy_slots;
+        else if (!free_slot) free_slot = &slot;
+    }
     if (!free_slot || srv->pending.empty()) return progress;
-    std::vector<SReq *> batch;
-    bool blocked = false;
-    int scanned = 0;
     // A batch takes requests until its work reaches batch_items_per_wave
     // items per resident wavefront: batches that end often let admission
     // (and with it the American requests waiting for arena memory) proceed
     // while the GPU computes, instead of one launch holding most of a burst.
     const int64_t cap = (int64_t)srv->cfg.batch_items_per_wave * srv->n_cu * srv->cfg.blocks_per_cu *
                         WAVES_PER_BLOCK * srv->cfg.item_path_steps;
+    // Beside a batch in flight, a second one fills the first one's tail (its
+    // last work items leave CUs idle until the host launches the next batch),
+    // but the two share the GPU and each takes longer.  With a backlog gate
+    // the overlap is used only while the queue holds that many batches' work
+    // (a burst), and light traffic keeps one batch at a time.
+    if (n_busy_slots > 0 && srv->cfg.overlap_min_backlog > 0 &&
+        srv->pending_ps < (int64_t)srv->cfg.overlap_min_backlog * cap)
+        return progress;
+    std::vector<SReq *> batch;
+    bool blocked = false;
+    int scanned = 0;
     int64_t batch_ps = 0;
+    auto dequeue = [&](std::deque<SReq *>::iterator it) {
+        srv->pending_ps -= (*it)->c.n_paths * (int64_t)(*it)->c.n_steps;
+        return srv->pending.erase(it);
+    };
     auto admit = [&](SReq *r, size_t off) {
         r->off = off;
         r->base = srv->arena.base;
@@ -649,13 +664,13 @@ bool schedule_once(qk_server *srv) {
         const size_t off = srv->arena.alloc(r->lay.total);
         if (off == SIZE_MAX) break;
         admit(r, off);
-        it = srv->pending.erase(it);
+        it = dequeue(it);
     }
     for (auto it = srv->pending.begin(); it != srv->pending.end() && scanned < ADMIT_SCAN;) {
         SReq *r = *it;
         if (!batch.empty() && batch_ps >= cap) break;
         if (r->lay.total > srv->arena.size) {
-            it = srv->pending.erase(it);
+            it = dequeue(it);
             r->err = "request footprint exceeds the server arena";
             complete(srv, r, QK_ENOMEM);
             srv->cv_done.notify_all();
@@ -676,7 +691,7 @@ bool schedule_once(qk_server *srv) {
             continue;
         }
         admit(r, off);
-        it = srv->pending.erase(it);
+        it = dequeue(it);
     }
     if (batch.empty()) return progress;
     const hipError_t e = launch_slot(srv, *free_slot, batch);
@@ -739,6 +754,7 @@ extern "C" qk_status qk_server_create(qk_ctx *ctx, const qk_server_config *cfg, 
     if (s->cfg.item_path_steps <= 0) s->cfg.item_path_steps = 131072;
     if (s->cfg.blocks_per_cu <= 0) s->cfg.blocks_per_cu = 4;
     if (s->cfg.batch_items_per_wave <= 0) s->cfg.batch_items_per_wave = 32;
+    if (s->cfg.overlap_min_backlog < 0) s->cfg.overlap_min_backlog = 0;
     int n_cu = 0;
     if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && n_cu > 0)
         s->n_cu = n_cu;
@@ -849,6 +865,7 @@ extern "C" qk_status qk_server_submit_ex(qk_server *s, const qk_contract *c, con
     }
     s->reqs[r->ticket] = r;
     s->pending.push_back(r);
+    s->pending_ps += c->n_paths * (int64_t)c->n_steps;
     ++s->n_submitted;
     *ticket = r->ticket;
     s->cv_work.notify_one();

@@ -28,6 +28,10 @@ def test_validate_rejects_bad_entries():
     bad["server"]["blocks_per_cu"] = -1
     with pytest.raises(ValueError):
         autotune.validate(bad)
+    bad = json.loads(json.dumps(good))
+    bad["server"]["overlap_min_backlog"] = -2
+    with pytest.raises(ValueError):
+        autotune.validate(bad)
 
 
 def test_load_round_trip_and_missing_file(tmp_path):

@@ -237,17 +237,20 @@ def test_batch_cap_and_american_first_do_not_change_results(engine, precision):
     """A batch work cap of one item per resident wavefront (a batch per few
     requests, American requests admitted ahead of earlier arrivals) and a
     small arena (American requests wait for memory) give the same bits as
-    the default schedule: a request's result depends only on (seed, path)."""
+    the default schedule: a request's result depends only on (seed, path).
+    The same holds with a second batch admitted only while the queue holds
+    two batches' work (overlap_min_backlog)."""
     cs, ps = small_stream(precision, n=40, seed=23)
     with Server(engine) as srv:
         ref = srv.replay(cs, ps, [0.0] * len(cs))
     rs = 4 if precision == qk.FP32 else 8
     need = max(c.n_paths * c.n_steps * rs for c, p in zip(cs, ps) if p.type == qk.AMERICAN)
-    with Server(engine, batch_items_per_wave=1, blocks_per_cu=2, max_inflight=2, item_path_steps=4096,
-                mem_budget_bytes=2 * need + (64 << 20)) as srv:
-        res = srv.replay(cs, ps, [0.0] * len(cs))
-        st = srv.stats()
-    assert st.n_batches > 2
-    for p, a, b in zip(ps, ref, res):
-        assert a.status == 0 and b.status == 0
-        assert same(a.stats, b.stats), (p.type, a.stats.price, b.stats.price)
+    for gate in (0, 2):
+        with Server(engine, batch_items_per_wave=1, blocks_per_cu=2, max_inflight=2, item_path_steps=4096,
+                    mem_budget_bytes=2 * need + (64 << 20), overlap_min_backlog=gate) as srv:
+            res = srv.replay(cs, ps, [0.0] * len(cs))
+            st = srv.stats()
+        assert st.n_batches > 2 and st.n_completed == len(cs), gate
+        for p, a, b in zip(ps, ref, res):
+            assert a.status == 0 and b.status == 0
+            assert same(a.stats, b.stats), (gate, p.type, a.stats.price, b.stats.price)
