@@ -296,9 +296,13 @@ def serving(engine, n_requests=4096, fp64_fraction=0.25):
         wall = time.perf_counter() - t0
         st = srv.stats()
     lat = sorted(r.latency_s for r in res)
-    q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))]
+    q = lambda v, f: v[min(len(v) - 1, int(f * len(v)))] * 1e3
+    lat_am = sorted(r.latency_s for r, p in zip(res, ps2) if p.type == qk.AMERICAN)
+    lat_rest = sorted(r.latency_s for r, p in zip(res, ps2) if p.type != qk.AMERICAN)
     out["poisson"] = {"rate_per_s": rate, "wall_s": wall, "path_steps_per_s": total_ps / wall,
-                      "p50_ms": q(0.50) * 1e3, "p99_ms": q(0.99) * 1e3, "max_ms": lat[-1] * 1e3,
+                      "p50_ms": q(lat, 0.50), "p99_ms": q(lat, 0.99), "max_ms": lat[-1] * 1e3,
+                      "american_p50_ms": q(lat_am, 0.50), "american_p99_ms": q(lat_am, 0.99),
+                      "other_p50_ms": q(lat_rest, 0.50), "other_p99_ms": q(lat_rest, 0.99),
                       "p50_queue_ms": sorted(r.queue_s for r in res)[len(res) // 2] * 1e3,
                       "batches": st.n_batches, "failed": sum(r.status != 0 for r in res)}
     # 3) plain per-request FP64 launches (the drop-in single-contract path)

@@ -17,10 +17,16 @@ LIB_PATH = os.path.join(PKG, "libquantkos.so")
 # a pump stream that shares a batch stream's queue waits behind every batch
 # kernel (measured on the 4,096-request stream: that chain's dates ~10x slower,
 # American p99 latency 224-344 ms instead of ~75 ms).  Eight queues give every
-# stream its own.  HIP reads the setting when it initialises, so it takes effect
-# when this package is imported before anything else in the process uses the
-# GPU; a value set by the caller is kept.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# stream its own, so a smaller setting (or none) is raised to 8.  HIP reads the
+# setting when it initialises, so it takes effect when this package is imported
+# before anything else in the process uses the GPU.
+MIN_HW_QUEUES = 8
+try:
+    _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+except ValueError:
+    _hwq = 0
+if _hwq < MIN_HW_QUEUES:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(MIN_HW_QUEUES)
 
 QK_OK, QK_EINVAL, QK_ENOMEM, QK_EDEVICE, QK_EAGAIN, QK_ENOTFOUND, QK_ENOTREADY = 0, -1, -2, -3, -4, -5, -6
 HESTON, BATES = 0, 1

@@ -82,3 +82,20 @@ def test_footprint_saturates_instead_of_wrapping():
     assert L.qk_footprint_bytes(C.byref(c), 1, 1, 0) == size_max
     c = qk.contract(n_paths=1 << 20, n_steps=100, precision=qk.FP32)
     assert L.qk_footprint_bytes(C.byref(c), 1, 1, 1) < 2 ** 30
+
+
+def test_package_raises_the_hardware_queue_count():
+    """Importing the package raises GPU_MAX_HW_QUEUES to 8 when it is unset or
+    lower (each of the request server's streams needs a hardware queue of its
+    own) and keeps a larger setting."""
+    import subprocess
+    import sys
+    code = "import os, quantkos_amd; print(os.environ['GPU_MAX_HW_QUEUES'])"
+    for before, after in (("4", "8"), (None, "8"), ("16", "16")):
+        env = dict(os.environ)
+        env.pop("GPU_MAX_HW_QUEUES", None)
+        if before is not None:
+            env["GPU_MAX_HW_QUEUES"] = before
+        out = subprocess.run([sys.executable, "-c", code], env=env, cwd=ROOT, capture_output=True, text=True,
+                             check=True).stdout.strip()
+        assert out == after, (before, out)
