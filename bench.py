@@ -279,13 +279,16 @@ def serving(engine, n_requests=4096, fp64_fraction=0.25):
     out["burst"] = {"wall_s": wall, "path_steps_per_s": cap, "batches": st.n_batches,
                     "arena_peak_gb": st.arena_peak_bytes / 2 ** 30}
     # the tuned schedule picks latency within 5 % of the best throughput; the
-    # throughput-best schedule (two batches in flight) beside it
-    progress("serving: burst %.2f s; burst with two batches in flight" % wall)
-    with qk.Server(engine, max_inflight=2) as srv:
+    # same schedule with the other number of batches in flight beside it
+    tuned_inflight = int(((engine.tuning or {}).get("server") or {}).get("max_inflight", 2) or 2)
+    other = 1 if tuned_inflight >= 2 else 2
+    progress("serving: burst %.2f s; burst with %d batch(es) in flight" % (wall, other))
+    with qk.Server(engine, max_inflight=other) as srv:
         t0 = time.perf_counter()
         srv.replay(cs, ps, [0.0] * len(cs))
         w2 = time.perf_counter() - t0
-    out["burst_two_in_flight"] = {"wall_s": w2, "path_steps_per_s": total_ps / w2}
+    out["burst"]["max_inflight"] = tuned_inflight
+    out["burst_other_in_flight"] = {"max_inflight": other, "wall_s": w2, "path_steps_per_s": total_ps / w2}
     # 2) Poisson arrivals at 0.8x the measured capacity
     rate = 0.8 * cap / mean_ps
     progress("serving: burst %.2f s, %.3g path-steps/s; Poisson at %.1f req/s" % (wall, cap, rate))

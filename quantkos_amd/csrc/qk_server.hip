@@ -166,7 +166,9 @@ struct qk_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<SReq *> pending;
-    int64_t pending_ps = 0;                    // path-steps of the pending requests
+    // Path-steps of the pending requests, in double: a request may hold up to
+    // 2^56, and a queue of them would overflow 64-bit integers.
+    double pending_ps = 0.0;
     std::unordered_map<uint64_t, SReq *> reqs;
     uint64_t next_ticket = 1;
     bool stop = false;
@@ -632,14 +634,14 @@ bool schedule_once(qk_server *srv) {
     // the overlap is used only while the queue holds that many batches' work
     // (a burst), and light traffic keeps one batch at a time.
     if (n_busy_slots > 0 && srv->cfg.overlap_min_backlog > 0 &&
-        srv->pending_ps < (int64_t)srv->cfg.overlap_min_backlog * cap)
+        srv->pending_ps < (double)srv->cfg.overlap_min_backlog * (double)cap)
         return progress;
     std::vector<SReq *> batch;
     bool blocked = false;
     int scanned = 0;
     int64_t batch_ps = 0;
     auto dequeue = [&](std::deque<SReq *>::iterator it) {
-        srv->pending_ps -= (*it)->c.n_paths * (int64_t)(*it)->c.n_steps;
+        srv->pending_ps -= (double)(*it)->c.n_paths * (double)(*it)->c.n_steps;
         return srv->pending.erase(it);
     };
     auto admit = [&](SReq *r, size_t off) {
@@ -865,7 +867,7 @@ extern "C" qk_status qk_server_submit_ex(qk_server *s, const qk_contract *c, con
     }
     s->reqs[r->ticket] = r;
     s->pending.push_back(r);
-    s->pending_ps += c->n_paths * (int64_t)c->n_steps;
+    s->pending_ps += (double)c->n_paths * (double)c->n_steps;
     ++s->n_submitted;
     *ticket = r->ticket;
     s->cv_work.notify_one();
