@@ -56,15 +56,18 @@ def test_default_lane_schedule_without_a_table():
     assert L.qk_set_schedule(None, None, 0) == -1
 
 
-def test_pick_server_prefers_latency_within_the_throughput_band():
+def test_pick_server_prefers_the_lower_p99_within_the_throughput_band():
     a, b, c = ("a",), ("b",), ("c",)
     rates = {a: 100.0, b: 98.0, c: 90.0}
-    # b is within 3 % of the best and has the lower p50*p99; c is outside the band
-    assert autotune.pick_server(rates, {a: (20.0, 200.0), b: (9.0, 230.0), c: (1.0, 10.0)}) == b
+    # b is within 3 % of the best and has the lower p99 (though a higher p50);
+    # c has the lowest latencies but is outside the band
+    assert autotune.pick_server(rates, {a: (9.0, 230.0), b: (20.0, 200.0), c: (1.0, 10.0)}) == b
+    # p99 within 5 % of the smallest: the higher throughput decides
+    assert autotune.pick_server(rates, {a: (20.0, 208.0), b: (9.0, 200.0)}) == a
     # without latency figures the fastest wins
     assert autotune.pick_server(rates, {}) == a
     # a tighter band excludes b
-    assert autotune.pick_server(rates, {a: (20.0, 200.0), b: (9.0, 230.0)}, tolerance=0.01) == a
+    assert autotune.pick_server(rates, {a: (9.0, 230.0), b: (20.0, 200.0)}, tolerance=0.01) == a
 
 
 def test_eight_lanes_per_thread_only_in_fp32():
