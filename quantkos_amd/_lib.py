@@ -11,16 +11,17 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libquantkos.so")
 
-# The request server runs up to two batch streams and four Longstaff-Schwartz
-# pump streams beside the ctx stream.  HIP maps streams onto a per-process pool
+# The request server runs up to two batch streams, an express-lane stream and
+# four Longstaff-Schwartz pump streams beside the ctx stream.  HIP maps
+# streams onto a per-process pool
 # of hardware queues (4 by default); streams beyond the pool share a queue, and
 # a pump stream that shares a batch stream's queue waits behind every batch
 # kernel (measured on the 4,096-request stream: that chain's dates ~10x slower,
-# American p99 latency 224-344 ms instead of ~75 ms).  Eight queues give every
-# stream its own, so a smaller setting (or none) is raised to 8.  HIP reads the
-# setting when it initialises, so it takes effect when this package is imported
-# before anything else in the process uses the GPU.
-MIN_HW_QUEUES = 8
+# American p99 latency 224-344 ms instead of ~75 ms).  Twelve queues give every
+# stream its own with room to spare, so a smaller setting (or none) is raised
+# to 12.  HIP reads the setting when it initialises, so it takes effect when
+# this package is imported before anything else in the process uses the GPU.
+MIN_HW_QUEUES = 12
 try:
     _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
 except ValueError:
@@ -112,7 +113,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("mem_budget_bytes", C.c_size_t), ("max_inflight", C.c_int32),
                 ("max_lsm_streams", C.c_int32), ("item_path_steps", C.c_int64),
                 ("blocks_per_cu", C.c_int32), ("batch_items_per_wave", C.c_int32),
-                ("overlap_min_backlog", C.c_int32)]
+                ("overlap_min_backlog", C.c_int32), ("express_max_path_steps", C.c_int64)]
 
 
 class ScheduleEntry(C.Structure):

@@ -1098,7 +1098,10 @@ __device__ __forceinline__ void batch_tile(const ReqD<Real> *r, int64_t tile) {
 
 template <typename Real>
 __global__ __launch_bounds__(BLOCK) void k_batch(const ReqD<Real> *__restrict__ reqs, const ItemD *__restrict__ items,
-                                                 int32_t n_items, unsigned int *counter) {
+                                                 int32_t n_items, unsigned int *counter, int32_t raised) {
+    // The server's express batches (small requests beside a large batch) take
+    // issue precedence over the large batch's wavefronts, below the pump's.
+    if (raised) __builtin_amdgcn_s_setprio(2);
     const int lid = threadIdx.x & 63;
     for (;;) {
         unsigned int v = 0;
@@ -1496,9 +1499,10 @@ hipError_t launch_pump(const PumpArgs<Real> &a, int n_basis, bool weighted, hipS
 
 template <typename Real>
 hipError_t launch_batch(const ReqD<Real> *reqs, const ItemD *items, int32_t n_items, unsigned int *counter,
-                        int blocks, hipStream_t s) {
+                        int blocks, bool raised, hipStream_t s) {
     if (n_items <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_batch<Real>, dim3(blocks), dim3(BLOCK), 0, s, reqs, items, n_items, counter);
+    hipLaunchKernelGGL(k_batch<Real>, dim3(blocks), dim3(BLOCK), 0, s, reqs, items, n_items, counter,
+                       (int32_t)raised);
     return hipGetLastError();
 }
 
@@ -1510,8 +1514,9 @@ hipError_t launch_batch_merge(const MergeReqD *mr, int n_reqs, const int32_t *ch
     return hipGetLastError();
 }
 
-template hipError_t launch_batch<float>(const ReqD<float> *, const ItemD *, int32_t, unsigned int *, int, hipStream_t);
-template hipError_t launch_batch<double>(const ReqD<double> *, const ItemD *, int32_t, unsigned int *, int,
+template hipError_t launch_batch<float>(const ReqD<float> *, const ItemD *, int32_t, unsigned int *, int, bool,
+                                        hipStream_t);
+template hipError_t launch_batch<double>(const ReqD<double> *, const ItemD *, int32_t, unsigned int *, int, bool,
                                          hipStream_t);
 template hipError_t launch_paths<float>(const PathArgs<float> &, int, int, bool, bool, int, hipStream_t);
 template hipError_t launch_paths<double>(const PathArgs<double> &, int, int, bool, bool, int, hipStream_t);

@@ -195,7 +195,7 @@ struct MergeReqD {
 
 template <typename Real>
 hipError_t launch_batch(const ReqD<Real> *reqs, const ItemD *items, int32_t n_items, unsigned int *counter,
-                        int blocks, hipStream_t s);
+                        int blocks, bool raised, hipStream_t s);
 hipError_t launch_batch_merge(const MergeReqD *mr, int n_reqs, const int32_t *chunk_req, int64_t n_chunks,
                               qk_stats *out, hipStream_t s);
 

@@ -245,7 +245,8 @@ class Server:
 
     def __init__(self, engine: Engine, mem_budget_bytes: int = 0, max_inflight: int = 0,
                  max_lsm_streams: int = 0, item_path_steps: int = 0, blocks_per_cu: int = 0,
-                 batch_items_per_wave: int = 0, overlap_min_backlog: int = None):
+                 batch_items_per_wave: int = 0, overlap_min_backlog: int = None,
+                 express_max_path_steps: int = None):
         self.engine = engine
         self.L = engine.L
         tuned = (engine.tuning or {}).get("server") or {}
@@ -255,8 +256,10 @@ class Server:
         batch_items_per_wave = batch_items_per_wave or int(tuned.get("batch_items_per_wave", 0))
         if overlap_min_backlog is None:       # 0 is a setting of its own (overlap whenever a stream is free)
             overlap_min_backlog = int(tuned.get("overlap_min_backlog", 0))
+        if express_max_path_steps is None:    # 0: the library's default lane; < 0: none
+            express_max_path_steps = int(tuned.get("express_max_path_steps", 0))
         cfg = ServerConfig(mem_budget_bytes, max_inflight, max_lsm_streams, item_path_steps, blocks_per_cu,
-                           batch_items_per_wave, overlap_min_backlog)
+                           batch_items_per_wave, overlap_min_backlog, express_max_path_steps)
         h = C.c_void_p()
         engine._check(self.L.qk_server_create(engine.h, C.byref(cfg), C.byref(h)))
         self.h = h

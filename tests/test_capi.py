@@ -85,13 +85,13 @@ def test_footprint_saturates_instead_of_wrapping():
 
 
 def test_package_raises_the_hardware_queue_count():
-    """Importing the package raises GPU_MAX_HW_QUEUES to 8 when it is unset or
+    """Importing the package raises GPU_MAX_HW_QUEUES to 12 when it is unset or
     lower (each of the request server's streams needs a hardware queue of its
     own) and keeps a larger setting."""
     import subprocess
     import sys
     code = "import os, quantkos_amd; print(os.environ['GPU_MAX_HW_QUEUES'])"
-    for before, after in (("4", "8"), (None, "8"), ("16", "16")):
+    for before, after in (("4", "12"), (None, "12"), ("16", "16")):
         env = dict(os.environ)
         env.pop("GPU_MAX_HW_QUEUES", None)
         if before is not None:
