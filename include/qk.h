@@ -292,10 +292,11 @@ typedef struct {
     int32_t overlap_min_backlog; /* a batch is launched beside one already in flight only while
                                     the pending requests hold at least this many batches' work
                                     (the work cap above); 0: whenever a stream is free */
-    int64_t express_max_path_steps; /* express lane: while its stream is free, pending
-                                       non-American requests of at most this many path-steps are
-                                       batched ahead of arrival order (at most 4x this in all) and
-                                       launched beside the running batch (0: 2^27; < 0: no lane) */
+    int64_t express_max_path_steps; /* express lane: while its stream is free, pending requests
+                                       of at most this many path-steps (an American request's path
+                                       store generation) are batched ahead of arrival order (at
+                                       most 4x this in all) and launched beside the running batch
+                                       (0: 2^27; < 0: no lane) */
 } qk_server_config;
 
 typedef struct {

@@ -611,11 +611,13 @@ void launch_admitted(qk_server *srv, Slot &slot, std::vector<SReq *> &batch) {
 // median request (~5e7 path-steps, well under a millisecond of the GPU)
 // would take two batch lengths (measured p50 15.5 ms on the 4,096-request
 // stream; 2.2-2.6 ms with the lane at 2^27 path-steps, p99 and burst
-// throughput unchanged).  While the express slot is free, pending
-// non-American requests of at most express_max_path_steps path-steps are
-// taken out of arrival order into a batch of their own, of at most
-// EXPRESS_BATCH_FACTOR times that size in all, launched beside the large
-// batch at raised issue priority.  Returns true if it launched a batch.
+// throughput unchanged).  While the express slot is free, pending requests
+// of at most express_max_path_steps path-steps are taken out of arrival
+// order into a batch of their own, of at most EXPRESS_BATCH_FACTOR times
+// that size in all, launched beside the large batch at raised issue
+// priority; an American request's path store is generated there and its
+// backward sweep then joins the pump (American p50 16-18 -> 8-9 ms).
+// Returns true if it launched a batch.
 constexpr int64_t EXPRESS_BATCH_FACTOR = 4;
 bool admit_express(qk_server *srv) {
     const int64_t lim = srv->cfg.express_max_path_steps;
@@ -633,7 +635,7 @@ bool admit_express(qk_server *srv) {
     for (auto it = srv->pending.begin(); it != srv->pending.end() && scanned < ADMIT_SCAN; ++scanned) {
         SReq *r = *it;
         const int64_t ps = r->c.n_paths * (int64_t)r->c.n_steps;
-        if (r->american || ps > lim || r->lay.total > srv->arena.size / 64) {
+        if (ps > lim || r->lay.total > srv->arena.size / 64) {
             ++it;
             continue;
         }
