@@ -1127,7 +1127,26 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const ReqD<Real> *__restrict__ 
 
 // Batched two-stage merge: stage 1 over a flattened (request, chunk) list,
 // stage 2 one block per request; both in fixed order within a request.
-__global__ __launch_bounds__(256) void k_bmerge_chunks(const MergeReqD *__restrict__ mr, const int32_t *__restrict__ chunk_req) {
+// Price, SE and P(ITM) of a request's merged moments (OptionPricer.hpp:204-249).
+__device__ __forceinline__ qk_stats bmerge_result(const Mom &a, double factor) {
+    const double N = a.n;
+    const double var = N > 1.0 ? a.m2 / (N - 1.0) : 0.0;
+    qk_stats o;
+    o.price = a.mean * factor;
+    o.se = sqrt(var / N) * factor;
+    o.prob_itm = (double)a.itm / N;
+    o.mean = a.mean;
+    o.m2 = a.m2;
+    o.n = (int64_t)N;
+    o.itm = (int64_t)a.itm;
+    return o;
+}
+
+// Stage 1 also finalises a request of a single chunk: stage 2 would merge
+// that chunk into an empty accumulator, which copies it (chan_merge), so the
+// result has the same bits, and stage 2 skips the request.
+__global__ __launch_bounds__(256) void k_bmerge_chunks(const MergeReqD *__restrict__ mr, const int32_t *__restrict__ chunk_req,
+                                                       qk_stats *__restrict__ out) {
     const int req = chunk_req[blockIdx.x];
     const MergeReqD m = mr[req];
     const int64_t chunk = blockIdx.x - m.first_chunk;
@@ -1144,11 +1163,15 @@ __global__ __launch_bounds__(256) void k_bmerge_chunks(const MergeReqD *__restri
         }
     }
     a = block_tree_merge(a);
-    if (threadIdx.x == 0) ((Mom *)m.scratch)[chunk] = a;
+    if (threadIdx.x == 0) {
+        if (m.n_chunks == 1) out[req] = bmerge_result(a, m.factor);
+        else ((Mom *)m.scratch)[chunk] = a;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bmerge_final(const MergeReqD *__restrict__ mr, qk_stats *__restrict__ out) {
     const MergeReqD m = mr[blockIdx.x];
+    if (m.n_chunks == 1) return;                     // finalised by stage 1 (uniform per block)
     Mom a{0.0, 0.0, 0.0, 0ull};
     for (int64_t c = threadIdx.x; c < m.n_chunks; c += 256) {
         const Mom x = ((const Mom *)m.scratch)[c];
@@ -1156,19 +1179,7 @@ __global__ __launch_bounds__(256) void k_bmerge_final(const MergeReqD *__restric
         a.itm += x.itm;
     }
     a = block_tree_merge(a);
-    if (threadIdx.x == 0) {
-        const double N = a.n;
-        const double var = N > 1.0 ? a.m2 / (N - 1.0) : 0.0;
-        qk_stats o;
-        o.price = a.mean * m.factor;
-        o.se = sqrt(var / N) * m.factor;
-        o.prob_itm = (double)a.itm / N;
-        o.mean = a.mean;
-        o.m2 = a.m2;
-        o.n = (int64_t)N;
-        o.itm = (int64_t)a.itm;
-        out[blockIdx.x] = o;
-    }
+    if (threadIdx.x == 0) out[blockIdx.x] = bmerge_result(a, m.factor);
 }
 
 // ---------------------------------------------------------------------------
@@ -1509,8 +1520,10 @@ hipError_t launch_batch(const ReqD<Real> *reqs, const ItemD *items, int32_t n_it
 hipError_t launch_batch_merge(const MergeReqD *mr, int n_reqs, const int32_t *chunk_req, int64_t n_chunks,
                               qk_stats *out, hipStream_t s) {
     if (n_reqs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bmerge_chunks, dim3((unsigned)n_chunks), dim3(256), 0, s, mr, chunk_req);
-    hipLaunchKernelGGL(k_bmerge_final, dim3(n_reqs), dim3(256), 0, s, mr, out);
+    hipLaunchKernelGGL(k_bmerge_chunks, dim3((unsigned)n_chunks), dim3(256), 0, s, mr, chunk_req, out);
+    // every request has at least one chunk: as many chunks as requests means
+    // one each, all finalised by stage 1
+    if (n_chunks > n_reqs) hipLaunchKernelGGL(k_bmerge_final, dim3(n_reqs), dim3(256), 0, s, mr, out);
     return hipGetLastError();
 }
 
