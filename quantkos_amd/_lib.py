@@ -19,14 +19,15 @@ LIB_PATH = os.path.join(PKG, "libquantkos.so")
 # kernel (measured on the 4,096-request stream: that chain's dates ~10x slower,
 # American p99 latency 224-344 ms instead of ~75 ms).  Twelve queues give every
 # stream its own with room to spare, so a smaller setting (or none) is raised
-# to 12.  HIP reads the setting when it initialises, so it takes effect when
-# this package is imported before anything else in the process uses the GPU.
+# to 12 unless QK_KEEP_HW_QUEUES=1.  HIP reads the setting when it
+# initialises, so it takes effect when this package is imported before
+# anything else in the process uses the GPU.
 MIN_HW_QUEUES = 12
 try:
     _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
 except ValueError:
     _hwq = 0
-if _hwq < MIN_HW_QUEUES:
+if _hwq < MIN_HW_QUEUES and os.environ.get("QK_KEEP_HW_QUEUES") != "1":
     os.environ["GPU_MAX_HW_QUEUES"] = str(MIN_HW_QUEUES)
 
 QK_OK, QK_EINVAL, QK_ENOMEM, QK_EDEVICE, QK_EAGAIN, QK_ENOTFOUND, QK_ENOTREADY = 0, -1, -2, -3, -4, -5, -6

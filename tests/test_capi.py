@@ -87,15 +87,18 @@ def test_footprint_saturates_instead_of_wrapping():
 def test_package_raises_the_hardware_queue_count():
     """Importing the package raises GPU_MAX_HW_QUEUES to 12 when it is unset or
     lower (each of the request server's streams needs a hardware queue of its
-    own) and keeps a larger setting."""
+    own), keeps a larger setting, and leaves it alone with QK_KEEP_HW_QUEUES=1."""
     import subprocess
     import sys
     code = "import os, quantkos_amd; print(os.environ['GPU_MAX_HW_QUEUES'])"
-    for before, after in (("4", "12"), (None, "12"), ("16", "16")):
+    for before, keep, after in (("4", None, "12"), (None, None, "12"), ("16", None, "16"), ("4", "1", "4")):
         env = dict(os.environ)
         env.pop("GPU_MAX_HW_QUEUES", None)
+        env.pop("QK_KEEP_HW_QUEUES", None)
         if before is not None:
             env["GPU_MAX_HW_QUEUES"] = before
+        if keep is not None:
+            env["QK_KEEP_HW_QUEUES"] = keep
         out = subprocess.run([sys.executable, "-c", code], env=env, cwd=ROOT, capture_output=True, text=True,
                              check=True).stdout.strip()
-        assert out == after, (before, out)
+        assert out == after, (before, keep, out)
