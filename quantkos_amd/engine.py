@@ -241,7 +241,12 @@ class Engine:
 
 
 class Server:
-    """A continuous-batching request server (qk_server_*) on an Engine's device."""
+    """A continuous-batching request server (qk_server_*) on an Engine's device.
+
+    The keyword arguments are the fields of qk_server_config (include/qk.h);
+    a 0 (or None for the last two) takes the engine's tuning table entry, then
+    the library default.  Every schedule gives the same bits for a request:
+    results depend only on the request itself."""
 
     def __init__(self, engine: Engine, mem_budget_bytes: int = 0, max_inflight: int = 0,
                  max_lsm_streams: int = 0, item_path_steps: int = 0, blocks_per_cu: int = 0,
