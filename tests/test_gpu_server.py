@@ -50,11 +50,11 @@ def test_server_matches_plain_launch(engine, precision):
 @pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
 def test_server_step_remainders_match_plain_launch(engine, precision):
     """The batch kernel's step loops for every remainder of the Philox block
-    sharing (QE: four steps per three blocks; Euler: two steps per block), for
-    path-dependent and European payoffs, against the plain launch."""
+    sharing (QE: four steps per three blocks; Euler and Milstein: two steps per
+    block), for path-dependent and European payoffs, against the plain launch."""
     cs, ps = [], []
     for i, n in enumerate((1, 2, 3, 5, 6, 7, 50, 101)):
-        for scheme in (qk.QE, qk.EULER):
+        for scheme in (qk.QE, qk.EULER, qk.MILSTEIN):
             for model in (qk.HESTON, qk.BATES):
                 cs.append(qk.contract(model=model, scheme=scheme, precision=precision, n_steps=n,
                                       n_paths=3_000 + 64 * i + 7, seed=100 + i, lambda_j=0.5))
@@ -260,3 +260,31 @@ def test_batch_cap_and_american_first_do_not_change_results(engine, precision):
         for p, a, b in zip(ps, ref, res):
             assert a.status == 0 and b.status == 0
             assert same(a.stats, b.stats), (kw, p.type, a.stats.price, b.stats.price)
+
+
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+def test_every_payoff_type_through_the_server(engine, precision):
+    """Each of the ten payoff types, call and put, submitted together to the
+    server (normal batches and the express lane) equals its plain launch (FP64
+    to 1e-12 with equal ITM counts, FP32 to 1e-5)."""
+    types = (qk.EUROPEAN, qk.ASIAN, qk.BARRIER_UP_OUT, qk.BARRIER_DOWN_OUT, qk.BARRIER_UP_IN,
+             qk.BARRIER_DOWN_IN, qk.LOOKBACK_FLOATING, qk.LOOKBACK_FIXED, qk.BINARY_CASH, qk.BINARY_ASSET)
+    cs, ps = [], []
+    for i, t in enumerate(types):
+        for right in (qk.CALL, qk.PUT):
+            B = 1.2 if t in (qk.BARRIER_UP_OUT, qk.BARRIER_UP_IN) else 0.85
+            cs.append(qk.contract(model=qk.HESTON if i % 2 else qk.BATES, scheme=qk.QE, precision=precision,
+                                  n_steps=52, n_paths=20_000 + 448 * i + 64 * right + 5, seed=300 + 2 * i + right,
+                                  lambda_j=0.3))
+            ps.append(qk.payoff(t, right, 1.05 if right == qk.CALL else 0.95, B))
+    with Server(engine) as srv:
+        res = srv.replay(cs, ps, [0.0] * len(cs))
+    tol = 1e-12 if precision == qk.FP64 else 1e-5
+    for c, p, r in zip(cs, ps, res):
+        assert r.status == 0, (p.type, p.right)
+        ref = plain(engine, c, p)
+        assert r.stats.n == c.n_paths, (p.type, p.right)
+        if precision == qk.FP64:
+            assert r.stats.itm == ref.itm, (p.type, p.right)
+        assert math.isclose(r.stats.price, ref.price, rel_tol=tol, abs_tol=1e-12), (p.type, p.right, r.stats.price,
+                                                                                     ref.price)
