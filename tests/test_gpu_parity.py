@@ -205,8 +205,13 @@ def test_lanes_match_single_launches(engine, precision, scheme, n_steps):
             assert math.isclose(a.price, b.price, rel_tol=tol, abs_tol=1e-12), (a.price, b.price)
 
 
-def test_greeks_fp64_match_oracle(engine, oracle):
-    c = base_contract(qk.HESTON, qk.MILSTEIN, qk.FP64, n_paths=20_000, n_steps=50, price_scale=100.0)
+@pytest.mark.parametrize("model,scheme", [(qk.HESTON, qk.MILSTEIN), (qk.HESTON, qk.QE), (qk.BATES, qk.EULER),
+                                          (qk.BATES, qk.QE)])
+def test_greeks_fp64_match_oracle(engine, oracle, model, scheme):
+    """The twelve-pricing stencil as lanes of one launch (each model and scheme
+    has its own lane kernels and tuned lanes per thread) against the oracle's
+    sequential repricings on the same random numbers."""
+    c = base_contract(model, scheme, qk.FP64, n_paths=20_000, n_steps=50, price_scale=100.0)
     b = qe.bumps(rel_S=0.01, abs_vol=0.02, abs_r=0.001, abs_T=0.02)
     pays = [qe.payoff(qk.EUROPEAN, qk.CALL, 1.0), qe.payoff(qk.BINARY_CASH, qk.CALL, 1.05)]
     g = engine.greeks(c, pays, b)
