@@ -231,6 +231,18 @@ def test_american_matches_oracle(engine, oracle, precision, model, scheme):
     conditioned (X = S/K clusters), so summation-order rounding moves individual
     coefficients; the fitted continuation values and the price must agree:
     FP64 within 0.02 SE, FP32 (FP32 uniforms, FP32 paths) within 0.5 SE."""
+    american_vs_oracle(engine, oracle, precision, model, scheme)
+
+
+@pytest.mark.parametrize("model,scheme", [(qk.HESTON, qk.EULER), (qk.HESTON, qk.QE), (qk.BATES, qk.EULER),
+                                          (qk.BATES, qk.MILSTEIN)])
+def test_american_fp64_other_schemes_match_oracle(engine, oracle, model, scheme):
+    """The path-store kernels of the other model and scheme pairs, FP64, by the
+    same checks as above."""
+    american_vs_oracle(engine, oracle, qk.FP64, model, scheme)
+
+
+def american_vs_oracle(engine, oracle, precision, model, scheme):
     c = base_contract(model, scheme, precision, n_paths=20_000, n_steps=50, price_scale=100.0)
     g, gco = engine.price_american(c, qk.PUT, 1.1, want_coeffs=True)
     o, oco = oracle.price_american(omodel(oracle, c), c.n_paths, qk.PUT, 1.1, 100.0, want_coeffs=True)
