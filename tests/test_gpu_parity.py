@@ -119,7 +119,7 @@ def all_payoffs(right):
             P(qk.BINARY_ASSET, right, 0.95)]
 
 
-@pytest.mark.parametrize("model,scheme", [(qk.HESTON, qk.QE), (qk.BATES, qk.EULER)])
+@pytest.mark.parametrize("model,scheme", MODELS)
 @pytest.mark.parametrize("right", [qk.CALL, qk.PUT])
 def test_prices_fp64_match_oracle(engine, oracle, model, scheme, right):
     c = base_contract(model, scheme, qk.FP64, n_paths=20_011, n_steps=50, price_scale=100.0)
