@@ -687,6 +687,7 @@ bool schedule_once(qk_server *srv) {
     }
     if (pump(srv)) progress = true;
     if (progress) srv->cv_done.notify_all();
+    if (admit_express(srv)) progress = true;
     // Admission while memory and a batch slot are free: American requests
     // first (below), then the rest in arrival order.  A request that does
     // not fit the free memory stops admission of requests of its size, so
@@ -695,7 +696,6 @@ bool schedule_once(qk_server *srv) {
     // moments only, released when their batch ends) may still go, so the
     // GPU keeps computing while the blocked request waits for memory that
     // the Longstaff-Schwartz pump is freeing.
-    if (admit_express(srv)) progress = true;
     Slot *free_slot = nullptr;
     int n_busy_slots = 0;
     for (Slot &slot : srv->slots) {
