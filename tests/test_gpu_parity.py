@@ -54,6 +54,26 @@ def test_paths_fp64_match_oracle(engine, oracle, model, scheme):
     assert np.median(rel) < 1e-14, np.median(rel)
 
 
+@pytest.mark.parametrize("scheme", [qk.EULER, qk.QE])
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+def test_bates_many_jumps_per_step_match_oracle(engine, oracle, scheme, precision):
+    """Bates at lambda dt = 5: several jumps per step, so the Poisson loop
+    draws uniforms e_j from word j mod 4 of the extra blocks (path, step,
+    1 + j/4) well past the first, and the jump normal from the words after
+    the last uniform.  Per-path spots equal the oracle's (FP64 1e-12; FP32
+    the per-path tolerance of the FP32 parity test)."""
+    c = base_contract(qk.BATES, scheme, precision, n_steps=20, n_paths=640, lambda_j=100.0, mu_j=-0.01,
+                      sigma_j=0.02, T=1.0)
+    g = engine.simulate_paths(c)
+    o = oracle.simulate(omodel(oracle, c), c.n_paths, store=True)["paths"]
+    rel = np.abs(g - o) / np.abs(o)
+    assert np.isfinite(g).all()
+    if precision == qk.FP64:
+        assert rel.max() < 1e-12, (rel.max(), np.unravel_index(rel.argmax(), rel.shape))
+    else:
+        assert np.median(rel) < 5e-6 and (rel > 1e-4).mean() < 0.01, (np.median(rel), (rel > 1e-4).mean())
+
+
 @pytest.mark.parametrize("path0", [37, 2 ** 32 - 100])
 def test_paths_at_counter_offsets_match_oracle(engine, oracle, path0):
     """qk_simulate_paths over [path0, path0 + 200): the random stream is keyed by
