@@ -607,3 +607,26 @@ def test_zero_jump_intensity_never_jumps(engine, oracle, scheme):
     assert np.array_equal(a, b)
     o = oracle.simulate(omodel(oracle, c), c.n_paths, path0=path0, store=True)["paths"]
     assert np.median(np.abs(a - o) / np.abs(o)) < 5e-6
+
+
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+@pytest.mark.parametrize("right", [qk.CALL, qk.PUT])
+def test_barrier_in_out_parity(engine, precision, right):
+    """Knock-in plus knock-out is the vanilla payoff on every path (Payoff.hpp:
+    157-242: exactly one of the pair pays), so on one path set the in and out
+    prices add up to the European price and their ITM counts to its count;
+    with the barrier at S0 (the running max and min start at S0) the out
+    option pays nothing and the in option is the European one."""
+    c = base_contract(qk.BATES, qk.QE, precision, n_paths=200_003, n_steps=100, price_scale=100.0)
+    P = qe.payoff
+    for up, dn in ((1.15, 0.88), (1.0, 1.0)):
+        pays = [P(qk.EUROPEAN, right, 1.0), P(qk.BARRIER_UP_OUT, right, 1.0, up), P(qk.BARRIER_UP_IN, right, 1.0, up),
+                P(qk.BARRIER_DOWN_OUT, right, 1.0, dn), P(qk.BARRIER_DOWN_IN, right, 1.0, dn)]
+        eu, uo, ui, do, di = engine.price_payoffs(c, pays)
+        for o_, i_ in ((uo, ui), (do, di)):
+            assert o_.itm + i_.itm == eu.itm, (up, dn, o_.itm, i_.itm, eu.itm)
+            assert math.isclose(o_.price + i_.price, eu.price, rel_tol=1e-10, abs_tol=1e-12), \
+                (up, dn, o_.price, i_.price, eu.price)
+        if up == 1.0:
+            assert uo.price == 0.0 and uo.itm == 0 and do.price == 0.0 and do.itm == 0
+            assert (ui.price, ui.itm) == (eu.price, eu.itm) and (di.price, di.itm) == (eu.price, eu.itm)
