@@ -630,3 +630,21 @@ def test_barrier_in_out_parity(engine, precision, right):
         if up == 1.0:
             assert uo.price == 0.0 and uo.itm == 0 and do.price == 0.0 and do.itm == 0
             assert (ui.price, ui.itm) == (eu.price, eu.itm) and (di.price, di.itm) == (eu.price, eu.itm)
+
+
+@pytest.mark.parametrize("precision", [qk.FP64, qk.FP32])
+def test_vanilla_is_asset_minus_cash_binary(engine, precision):
+    """(S - K)+ = S 1{S > K} - K 1{S > K} and (K - S)+ = K 1{S < K} - S 1{S < K}
+    on every path, so on one path set the European price is the asset-or-
+    nothing price minus K times the cash-or-nothing one (the cash binary is not
+    scaled by S0: Normalize_prices, OptionPricer.hpp:323-332)."""
+    scale = 100.0
+    c = base_contract(qk.HESTON, qk.MILSTEIN, precision, n_paths=200_003, n_steps=64, price_scale=scale)
+    P = qe.payoff
+    for right, K in ((qk.CALL, 1.05), (qk.PUT, 0.95)):
+        eu, asset, cash = engine.price_payoffs(c, [P(qk.EUROPEAN, right, K), P(qk.BINARY_ASSET, right, K),
+                                                  P(qk.BINARY_CASH, right, K)])
+        sign = 1.0 if right == qk.CALL else -1.0
+        Kr = float(np.float32(K)) if precision == qk.FP32 else K    # the strike as the kernel holds it
+        assert math.isclose(eu.price, sign * (asset.price - scale * Kr * cash.price), rel_tol=1e-9, abs_tol=1e-9), \
+            (right, eu.price, asset.price, cash.price)
