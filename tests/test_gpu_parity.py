@@ -648,3 +648,29 @@ def test_vanilla_is_asset_minus_cash_binary(engine, precision):
         Kr = float(np.float32(K)) if precision == qk.FP32 else K    # the strike as the kernel holds it
         assert math.isclose(eu.price, sign * (asset.price - scale * Kr * cash.price), rel_tol=1e-9, abs_tol=1e-9), \
             (right, eu.price, asset.price, cash.price)
+
+
+@pytest.mark.parametrize("precision", [qk.FP32, qk.FP64])
+def test_config1_full_size_properties(engine, precision):
+    """The headline workload at its full size (Heston QE, 4,000,000 paths x
+    365 steps, S0 = 100 normalised): properties that hold path by path and so
+    at any size.  Up-and-out plus up-and-in is the European call (prices add,
+    ITM counts add); call minus put is the asset-or-nothing call at a strike
+    near 0 minus K e^{-rT}; and that asset price is the forward discounted,
+    S0 e^{-qT}, within 3 standard errors (martingale)."""
+    c = qe.contract(model=qk.HESTON, scheme=qk.QE, precision=precision, n_steps=365, n_paths=4_000_000,
+                    seed=42, S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5,
+                    rho=-0.7, price_scale=100.0)
+    P = qe.payoff
+    K0 = 1e-6
+    eu, uo, ui, put, asset0 = engine.price_payoffs(c, [
+        P(qk.EUROPEAN, qk.CALL, 1.0), P(qk.BARRIER_UP_OUT, qk.CALL, 1.0, 1.3), P(qk.BARRIER_UP_IN, qk.CALL, 1.0, 1.3),
+        P(qk.EUROPEAN, qk.PUT, 1.0), P(qk.BINARY_ASSET, qk.CALL, K0)])
+    assert uo.itm + ui.itm == eu.itm
+    assert math.isclose(uo.price + ui.price, eu.price, rel_tol=1e-10)
+    disc = math.exp(-c.r * c.T)
+    tol = 1e-7 if precision == qk.FP32 else 1e-10
+    assert math.isclose(eu.price - put.price, asset0.price - 100.0 * disc, rel_tol=tol, abs_tol=1e-9), \
+        (eu.price, put.price, asset0.price)
+    assert asset0.itm == c.n_paths
+    assert abs(asset0.price - 100.0 * math.exp(-c.q * c.T)) < 3.0 * asset0.se, (asset0.price, asset0.se)
