@@ -674,3 +674,29 @@ def test_config1_full_size_properties(engine, precision):
         (eu.price, put.price, asset0.price)
     assert asset0.itm == c.n_paths
     assert abs(asset0.price - 100.0 * math.exp(-c.q * c.T)) < 3.0 * asset0.se, (asset0.price, asset0.se)
+
+
+def test_config2_full_size_properties(engine):
+    """The American workload at its full size (Bates QE with Merton jumps,
+    put K = 110 on S0 = 100, 500,000 paths, 50 exercise dates): FP32 and FP64
+    on the same random numbers agree within half a standard error, the price
+    is at least the European one and at least the intrinsic value, and the
+    per-date and cooperative backward passes give the same bits."""
+    out = {}
+    for prec in (qk.FP32, qk.FP64):
+        c = qe.contract(model=qk.BATES, scheme=qk.QE, precision=prec, n_steps=50, n_paths=500_000, seed=42,
+                        S0=1.0, v0=0.04, r=0.03, q=0.0, T=1.0, kappa=2.0, theta=0.04, sigma=0.5, rho=-0.7,
+                        lambda_j=0.1, mu_j=-0.05, sigma_j=0.10, price_scale=100.0)
+        am = engine.price_american(c, qk.PUT, 1.1)
+        eu = engine.price(c, qe.payoff(qk.EUROPEAN, qk.PUT, 1.1))
+        assert am.n == c.n_paths
+        assert am.price >= eu.price - 3.0 * eu.se and am.price >= 10.0 - 1e-9, (am.price, eu.price)
+        engine.set_lsm_schedule(qk._lib.LSM_COOPERATIVE)
+        try:
+            co = engine.price_american(c, qk.PUT, 1.1)
+        finally:
+            engine.set_lsm_schedule(qk._lib.LSM_AUTO)
+        assert (co.price, co.se, co.itm, co.mean, co.m2) == (am.price, am.se, am.itm, am.mean, am.m2)
+        out[prec] = am
+    a, b = out[qk.FP32], out[qk.FP64]
+    assert abs(a.price - b.price) < 0.5 * b.se, (a.price, b.price, b.se)
